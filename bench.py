@@ -1,0 +1,247 @@
+"""Benchmark of the MI355X fastvid codec on its headline workload.
+
+Workload (one "step"): encode then decode 16 frames of 3840x2160 RGB 4:4:4
+8-bit (synthetic, seeds 1-16), quality 90 (quantiser step 3), MED predictor,
+256x128 tiles, as one batched launch sequence per direction; frames resident
+in HBM before the timed region.  Metric: luma Gpixel/s of encode+decode, with
+per-frame latency percentiles of the plain per-frame call beside it.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build targets one
+GPU (frames are independent, see the sharding note in DESIGN.md).
+"""
+
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+WIDTH, HEIGHT, FRAMES, QUALITY, TILE = 3840, 2160, 16, 90, (256, 128)
+HBM_PEAK_GBS = 8000.0
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=FRAMES)
+    ap.add_argument("--latency-reps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-verify", action="store_true")
+    return ap.parse_args()
+
+
+def make_frames(count):
+    from fastvid_amd import synthetic
+    with cf.ThreadPoolExecutor(max_workers=min(8, count)) as pool:
+        return list(pool.map(lambda s: synthetic.rgb_frame(s, WIDTH, HEIGHT), range(1, count + 1)))
+
+
+def algorithmic_bytes(samples, stream_bytes):
+    """Per-kernel algorithmic bytes: residuals at their minimal width (u16
+    for bit depth <= 12), uint16 samples, stream bytes."""
+    s, b = samples, stream_bytes
+    return {
+        "k_wavefront_enc": 2 * s + 2 * s,   # samples in, residuals out
+        "k_analyze": 2 * s,                 # residuals in
+        "k_rans": 2 * s + b,                # residuals in, lane bytes out
+        "k_emit": 2 * s + 2 * b,            # residuals + lane bytes in, stream out
+        "k_entropy": b + 2 * s,             # stream in, residuals out
+        "k_wavefront_dec": 2 * s + 2 * s,   # residuals in, samples out
+    }
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; frames are independent "
+                                   "and shard across GPUs without collectives (see DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        return 2
+    import torch
+    from fastvid_amd import _native
+    from fastvid_amd.codec import DecodeJob, EncodeJob, frame_args, encode as enc1, decode as dec1
+
+    _native.lib()
+    device = torch.device("cuda", 0)
+    torch.cuda.set_device(device)
+    t0 = time.time()
+    host_frames = make_frames(args.frames)
+    gen_s = time.time() - t0
+    frames = [tuple(torch.from_numpy(p.astype(np.int32)).to(torch.uint16).to(device) for p in f)
+              for f in host_frames]
+    torch.cuda.synchronize()
+    fargs = [frame_args(f, layout="rgb444", bit_depth=8, quality=QUALITY, tile_size=TILE)
+             for f in frames]
+
+    # ---- one untimed pass for the decode plan and verification
+    ejob = EncodeJob(fargs).allocate()
+    ejob.launch()
+    streams = ejob.finish()
+    djob = DecodeJob(streams).allocate()
+    djob.launch()
+    decoded = djob.finish()
+    stream_bytes = int(sum(s.numel() for s in streams))
+
+    verify = {"checked": False}
+    if not args.no_verify:
+        from oracle import oracle as O
+        threads = len(os.sched_getaffinity(0))
+        ok_streams = True
+        ok_decode = True
+        for i, hf in enumerate(host_frames):
+            ref = O.encode(hf, layout="rgb444", bit_depth=8, quality=QUALITY, tile_size=TILE,
+                           threads=threads)
+            got = bytes(streams[i].cpu().numpy().tobytes())
+            ok_streams &= got == ref
+            if i < 2:
+                ref_planes = O.decode(ref, threads=threads)
+                ok_decode &= all(np.array_equal(a.cpu().numpy(), b)
+                                 for a, b in zip(decoded[i], ref_planes))
+        verify = {"checked": True, "streams_bit_exact": bool(ok_streams),
+                  "decode_bit_exact_sampled": bool(ok_decode)}
+
+    # ---- timed batched steps: encode batch + decode batch
+    def step():
+        ejob.launch()
+        ejob.finish()
+        djob.launch()
+        djob.finish()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _native.timing_read(reset=True)
+    _native.timing_enable(True)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    _native.timing_enable(False)
+    kernels = _native.timing_read(reset=True)
+    ms_per_step = elapsed * 1e3 / args.steps
+    luma = WIDTH * HEIGHT * args.frames
+    value = luma / (ms_per_step / 1e3) / 1e9
+
+    # encode-only / decode-only throughput of the batched form
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    enc_s = timed(lambda: (ejob.launch(), ejob.finish()), max(2, args.steps // 2))
+    dec_s = timed(lambda: (djob.launch(), djob.finish()), max(2, args.steps // 2))
+
+    # ---- plain per-frame calls (the reference's API shape): latency percentiles
+    lat_enc, lat_dec = [], []
+    for f in frames[:2]:
+        enc1(f, layout="rgb444", bit_depth=8, quality=QUALITY, tile_size=TILE)
+    for _ in range(args.latency_reps):
+        for f in frames:
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            s = enc1(f, layout="rgb444", bit_depth=8, quality=QUALITY, tile_size=TILE)
+            torch.cuda.synchronize()
+            lat_enc.append((time.perf_counter() - t) * 1e3)
+            t = time.perf_counter()
+            dec1(s)
+            torch.cuda.synchronize()
+            lat_dec.append((time.perf_counter() - t) * 1e3)
+
+    def pct(values, q):
+        return float(np.percentile(np.array(values), q))
+
+    plain_gpps = WIDTH * HEIGHT / (statistics.median(lat_enc) + statistics.median(lat_dec)) / 1e6
+
+    # ---- roofline of the dominant kernel (HIP events on the launch stream)
+    samples = 3 * WIDTH * HEIGHT * args.frames
+    alg = algorithmic_bytes(samples, stream_bytes)
+    stages = {}
+    for name, (launches, total_ms) in kernels.items():
+        per = total_ms / max(launches, 1)
+        entry = {"launches": launches, "avg_ms": round(per, 5)}
+        if name in alg:
+            launches_per_step = launches / args.steps
+            bytes_per_launch = alg[name] / launches_per_step
+            entry["gbs"] = round(bytes_per_launch / (per / 1e3) / 1e9, 1)
+            entry["hbm_frac"] = round(entry["gbs"] / HBM_PEAK_GBS, 4)
+        stages[name] = entry
+    dominant = max(kernels.items(), key=lambda kv: kv[1][1])[0] if kernels else None
+    roofline = None
+    if dominant in alg:
+        d = stages[dominant]
+        roofline = {"bound": "hbm", "kernel": dominant, "achieved": d["gbs"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": d["hbm_frac"], "traffic": None}
+
+    # ---- CPU baseline: the oracle on host cores, bounded sample
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = len(os.sched_getaffinity(0))
+        done, t = 0, time.perf_counter()
+        while time.perf_counter() - t < args.cpu_seconds or done == 0:
+            hf = host_frames[done % len(host_frames)]
+            s = O.encode(hf, layout="rgb444", bit_depth=8, quality=QUALITY, tile_size=TILE,
+                         threads=threads)
+            O.decode(s, threads=threads)
+            done += 1
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": round(WIDTH * HEIGHT * done / cpu_s / 1e9, 4), "unit": "Gpixel/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{done} frames of the same workload, encode+decode, "
+                         f"{cpu_s:.1f} s, C oracle with {threads} threads"}
+
+    line = {
+        "metric": "encode+decode Gpixel/s (luma), batched, bit-exact",
+        "value": round(value, 3),
+        "unit": "Gpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16",
+        "data": "synthetic",
+        "config": {"workload": "16x 3840x2160 rgb444 8-bit, quality 90 (step 3), MED, "
+                               "256x128 tiles; one step = batched encode + batched decode",
+                   "frames": args.frames, "width": WIDTH, "height": HEIGHT,
+                   "parallelism": "single GPU"},
+        "encode_gpps_batched": round(luma / enc_s / 1e9, 3),
+        "decode_gpps_batched": round(luma / dec_s / 1e9, 3),
+        "plain_per_frame": {"encode_ms_p50": round(pct(lat_enc, 50), 3),
+                            "encode_ms_p99": round(pct(lat_enc, 99), 3),
+                            "decode_ms_p50": round(pct(lat_dec, 50), 3),
+                            "decode_ms_p99": round(pct(lat_dec, 99), 3),
+                            "encode_decode_gpps": round(plain_gpps, 3)},
+        "stream_bytes_per_frame": stream_bytes // args.frames,
+        "verify": verify,
+        "roofline": roofline,
+        "stages": stages,
+        "cpu_baseline": cpu,
+        "setup_s": round(gen_s, 1),
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,329 @@
+"""Host orchestration of the MI355X codec: torch tensors in, torch tensors out.
+
+``encode``/``decode`` keep the signatures, defaults, layout inference and
+error behaviour of the reference extension API
+(cuda/fastvid_cuda/__init__.py:8-58, argument checks of
+cuda/csrc/encode.cu:695-725 and cuda/csrc/fastvid_cuda.cpp:141-183).
+``encode_batch``/``decode_batch`` run any number of frames in one set of
+launches.  PyTorch provides device memory and the current stream only; every
+byte of codec work happens in libfastvid_hip.so.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Sequence
+
+import torch
+
+from . import _native
+
+_LAYOUTS = {"gray": 0, "yuv422": 1, "rgb444": 2}
+
+
+@dataclass
+class FrameArgs:
+    planes: tuple
+    layout: int
+    bit_depth: int
+    quality: int
+    fps_numerator: int
+    fps_denominator: int
+    tile_width: int
+    tile_height: int
+
+
+def _layout_of(planes, layout):
+    # cuda/fastvid_cuda/__init__.py:28-33
+    if layout is None:
+        return 0 if len(planes) == 1 else (1 if planes[1].shape[1] != planes[0].shape[1] else 2)
+    if isinstance(layout, str):
+        return _LAYOUTS[layout]
+    return int(layout)
+
+
+def frame_args(planes, *, layout=None, bit_depth=10, quality=90, frame_rate=(24, 1),
+               tile_size=(256, 128)) -> FrameArgs:
+    if isinstance(planes, torch.Tensor):
+        planes = (planes,)
+    else:
+        planes = tuple(planes)
+    layout = _layout_of(planes, layout)
+    fps_numerator, fps_denominator = frame_rate
+    tile_width, tile_height = tile_size
+    return FrameArgs(planes, layout, int(bit_depth), int(quality), int(fps_numerator),
+                     int(fps_denominator), int(tile_width), int(tile_height))
+
+
+def _validate(a: FrameArgs) -> None:
+    """Argument checks in the order of encode.cu:695-725 (RuntimeError)."""
+    if not 0 <= a.layout <= 2:
+        raise RuntimeError("layout must be gray, yuv422, or rgb444")
+    if len(a.planes) != (1 if a.layout == 0 else 3):
+        raise RuntimeError("plane count does not match layout")
+    if a.bit_depth not in (8, 10, 12, 16):
+        raise RuntimeError("bit_depth must be 8, 10, 12, or 16")
+    if not 1 <= a.quality <= 100:
+        raise RuntimeError("quality must be in 1..=100")
+    if not (0 < a.fps_numerator <= 0xFFFFFFFF and 0 < a.fps_denominator <= 0xFFFFFFFF):
+        raise RuntimeError("frame-rate components must be nonzero u32 values")
+    if not (0 < a.tile_width <= 0xFFFF and 0 < a.tile_height <= 0xFFFF):
+        raise RuntimeError("tile dimensions must be nonzero u16 values")
+    first = a.planes[0]
+    if not isinstance(first, torch.Tensor) or not first.is_cuda:
+        raise RuntimeError("encoder input planes must be CUDA tensors")
+    if first.dtype != torch.uint16 or first.dim() != 2:
+        raise RuntimeError("encoder planes must be two-dimensional uint16 tensors")
+    height, width = first.shape
+    if not (0 < width <= 0xFFFFFFFF and 0 < height <= 0xFFFFFFFF):
+        raise RuntimeError("frame dimensions must be nonzero u32 values")
+    secondary = (width + 1) // 2 if a.layout == 1 else width
+    for index, plane in enumerate(a.planes):
+        if plane.device != first.device:
+            raise RuntimeError("all planes must use the same CUDA device")
+        if plane.dtype != torch.uint16 or plane.dim() != 2:
+            raise RuntimeError("encoder planes must be two-dimensional uint16 tensors")
+        expected = width if index == 0 else secondary
+        if plane.shape[0] != height or plane.shape[1] != expected:
+            raise RuntimeError("plane dimensions do not match declared layout")
+
+
+def _row_major(plane: torch.Tensor) -> torch.Tensor:
+    """Planes with unit column stride are read in place with their row pitch."""
+    if plane.stride(1) == 1 and plane.stride(0) >= plane.shape[1]:
+        return plane
+    return plane.contiguous()
+
+
+def _frame_struct(a: FrameArgs, keep: list) -> _native.EncodeFrame:
+    s = _native.EncodeFrame()
+    s.plane_count = len(a.planes)
+    for p, plane in enumerate(a.planes):
+        plane = _row_major(plane)
+        keep.append(plane)
+        s.planes[p] = plane.data_ptr()
+        s.pitches[p] = plane.stride(0)
+        s.plane_heights[p] = plane.shape[0]
+        s.plane_widths[p] = plane.shape[1]
+    s.layout = a.layout
+    s.bit_depth = a.bit_depth
+    s.quality = a.quality
+    s.fps_numerator = a.fps_numerator
+    s.fps_denominator = a.fps_denominator
+    s.tile_width = a.tile_width
+    s.tile_height = a.tile_height
+    return s
+
+
+class EncodeJob:
+    """Plans, allocates and launches the encode of a list of frames.
+
+    ``launch()`` only enqueues work on the current stream; ``finish()``
+    synchronises through the result read-back and returns the streams."""
+
+    def __init__(self, frames: Sequence[FrameArgs]):
+        if not frames:
+            raise ValueError("no frames to encode")
+        for a in frames:
+            _validate(a)
+        device = frames[0].planes[0].device
+        for a in frames:
+            if a.planes[0].device != device:
+                raise RuntimeError("all frames of a batch must use the same CUDA device")
+        self.device = device
+        self.frames = list(frames)
+        self._keep: list = []
+        count = len(frames)
+        self._structs = (_native.EncodeFrame * count)(
+            *[_frame_struct(a, self._keep) for a in frames])
+        lib = _native.lib()
+        msg = _native.message_buffer()
+        ws = ctypes.c_uint64()
+        out = ctypes.c_uint64()
+        self.offsets = (ctypes.c_uint64 * count)()
+        _native.check(lib.fvh_encode_plan(ctypes.addressof(self._structs), count,
+                                          ctypes.byref(ws), ctypes.byref(out), self.offsets,
+                                          msg, len(msg)), msg)
+        self.workspace_bytes = ws.value
+        self.output_bytes = out.value
+        self.workspace = None
+        self.output = None
+        self.results = None
+
+    def allocate(self, workspace=None, output=None, results=None):
+        n = len(self.frames)
+        self.workspace = workspace if workspace is not None else torch.empty(
+            max(self.workspace_bytes, 1), dtype=torch.uint8, device=self.device)
+        self.output = output if output is not None else torch.empty(
+            max(self.output_bytes, 1), dtype=torch.uint8, device=self.device)
+        self.results = results if results is not None else torch.empty(
+            n * _native.RESULT_BYTES, dtype=torch.uint8, device=self.device)
+        return self
+
+    def launch(self, stream=None):
+        if self.workspace is None:
+            self.allocate()
+        lib = _native.lib()
+        msg = _native.message_buffer()
+        with torch.cuda.device(self.device):
+            stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+            _native.check(lib.fvh_encode_launch(
+                ctypes.addressof(self._structs), len(self.frames), self.workspace.data_ptr(),
+                self.workspace.numel(), self.output.data_ptr(), self.output.numel(),
+                self.results.data_ptr(), ctypes.c_void_p(stream.cuda_stream), msg, len(msg)), msg)
+        return self
+
+    def finish(self):
+        host = self.results.cpu().numpy().tobytes()
+        streams = []
+        for i in range(len(self.frames)):
+            r = _native.Result.from_buffer_copy(host, i * _native.RESULT_BYTES)
+            if r.status != 0:
+                if r.status in (1, 2):
+                    raise RuntimeError(f"CUDA encoder analysis failed (status {r.status})")
+                if r.status == 100:
+                    raise RuntimeError("tile payload is too large")
+                raise RuntimeError(f"encoder failed (status {r.status})")
+            streams.append(self.output.narrow(0, int(self.offsets[i]), int(r.stream_length)))
+        return streams
+
+
+def encode_batch(frames: Sequence[FrameArgs]) -> list:
+    return EncodeJob(frames).allocate().launch().finish()
+
+
+def encode(planes, *, layout=None, bit_depth=10, quality=90, frame_rate=(24, 1),
+           tile_size=(256, 128)):
+    """Encode one uint16 gray, planar YUV 4:2:2 or RGB 4:4:4 frame on the GPU.
+
+    Same contract as the reference (cuda/fastvid_cuda/__init__.py:8-45):
+    ``planes`` is ``(Y,)`` or ``(Y, Cb, Cr)``; returns a one-dimensional uint8
+    version-1 bitstream on the planes' device."""
+    return encode_batch([frame_args(planes, layout=layout, bit_depth=bit_depth,
+                                    quality=quality, frame_rate=frame_rate,
+                                    tile_size=tile_size)])[0]
+
+
+# ------------------------------------------------------------------ decode
+
+def _read_header(encoded: torch.Tensor) -> _native.StreamInfo:
+    if encoded.dtype != torch.uint8:
+        raise RuntimeError("encoded must have dtype uint8")
+    if encoded.dim() != 1:
+        raise RuntimeError("encoded must be one-dimensional")
+    if not torch.cuda.is_available():
+        raise RuntimeError("CUDA is required")
+    size = encoded.numel()
+    head = encoded[: min(size, 32)]
+    head = head.cpu() if head.is_cuda else head
+    raw = bytes(head.contiguous().numpy().tobytes()) + b"\x00" * max(0, 32 - size)
+    info = _native.StreamInfo()
+    msg = _native.message_buffer()
+    buf = ctypes.create_string_buffer(raw, 32)
+    _native.check(_native.lib().fvh_decode_header(buf, size, ctypes.byref(info), msg, len(msg)),
+                  msg)
+    return info
+
+
+class DecodeJob:
+    """Validates, allocates and launches the decode of a list of streams."""
+
+    def __init__(self, streams: Sequence[torch.Tensor], wavefront: bool = True):
+        if not streams:
+            raise ValueError("no streams to decode")
+        lib = _native.lib()
+        self.infos = (_native.StreamInfo * len(streams))()
+        self.inputs = []
+        device = None
+        for i, encoded in enumerate(streams):
+            info = _read_header(encoded)
+            if encoded.is_cuda:
+                dev_stream = encoded.contiguous()
+            else:
+                host = encoded.contiguous()
+                msg = _native.message_buffer()
+                _native.check(lib.fvh_decode_check_host(
+                    ctypes.c_void_p(host.data_ptr()), host.numel(), ctypes.byref(info), msg,
+                    len(msg)), msg)
+                dev_stream = host.to(torch.device("cuda", torch.cuda.current_device()))
+            if device is None:
+                device = dev_stream.device
+            elif dev_stream.device != device:
+                raise RuntimeError("all streams of a batch must use the same CUDA device")
+            self.infos[i] = info
+            self.inputs.append(dev_stream)
+        self.device = device
+        self.wavefront = bool(wavefront)
+        msg = _native.message_buffer()
+        ws = ctypes.c_uint64()
+        _native.check(lib.fvh_decode_plan(ctypes.addressof(self.infos), len(streams),
+                                          ctypes.byref(ws), msg, len(msg)), msg)
+        self.workspace_bytes = ws.value
+        self.workspace = None
+        self.outputs = None
+        self.results = None
+
+    def allocate(self, workspace=None, outputs=None, results=None):
+        n = len(self.inputs)
+        self.workspace = workspace if workspace is not None else torch.empty(
+            max(self.workspace_bytes, 1), dtype=torch.uint8, device=self.device)
+        self.outputs = outputs if outputs is not None else [
+            torch.empty(self.infos[i].total_samples, dtype=torch.uint16, device=self.device)
+            for i in range(n)]
+        self.results = results if results is not None else torch.empty(
+            n * _native.RESULT_BYTES, dtype=torch.uint8, device=self.device)
+        return self
+
+    def launch(self, stream=None):
+        if self.workspace is None:
+            self.allocate()
+        n = len(self.inputs)
+        lib = _native.lib()
+        msg = _native.message_buffer()
+        ins = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.inputs])
+        outs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.outputs])
+        with torch.cuda.device(self.device):
+            stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+            _native.check(lib.fvh_decode_launch(
+                ctypes.addressof(self.infos), ins, outs, n, self.workspace.data_ptr(),
+                self.workspace.numel(), self.results.data_ptr(), int(self.wavefront),
+                ctypes.c_void_p(stream.cuda_stream), msg, len(msg)), msg)
+        return self
+
+    def planes(self, i: int):
+        info = self.infos[i]
+        out = self.outputs[i]
+        height, width = info.height, info.width
+        planes = [out.narrow(0, 0, info.y_samples).view(height, width)]
+        if info.layout != 0:
+            sw = info.secondary_samples // height
+            planes.append(out.narrow(0, info.y_samples, info.secondary_samples).view(height, sw))
+            planes.append(out.narrow(0, info.y_samples + info.secondary_samples,
+                                     info.secondary_samples).view(height, sw))
+        return tuple(planes)
+
+    def finish(self):
+        host = self.results.cpu().numpy().tobytes()
+        frames = []
+        for i in range(len(self.inputs)):
+            r = _native.Result.from_buffer_copy(host, i * _native.RESULT_BYTES)
+            if r.status != 0:
+                raise RuntimeError(f"malformed entropy payload (CUDA status {r.status})")
+            frames.append(self.planes(i))
+        return frames
+
+
+def decode_batch(streams: Sequence[torch.Tensor], wavefront: bool = True) -> list:
+    return DecodeJob(streams, wavefront).allocate().launch().finish()
+
+
+def decode(encoded, predictor="wavefront"):
+    """Decode one version-1 stream (uint8 tensor in host or device memory).
+
+    Same contract as the reference (cuda/fastvid_cuda/__init__.py:48-58):
+    returns ``(Y,)`` or ``(Y, Cb, Cr)`` contiguous uint16 planes on the
+    stream's device (the current device for a host stream)."""
+    if predictor not in ("wavefront", "serial"):
+        raise ValueError("predictor must be 'wavefront' or 'serial'")
+    return decode_batch([encoded], predictor == "wavefront")[0]

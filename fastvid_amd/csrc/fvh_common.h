@@ -1,0 +1,146 @@
+// fvh_common.h -- shared host/device definitions of the MI355X fastvid codec.
+//
+// Format constants follow the version-1 bitstream (cuda/csrc/encode.cu:14-28,
+// cuda/csrc/decode.cu:12-21).  Batch descriptors are this library's own
+// design: a batch is a flat list of frames; tiles and shards of all frames are
+// numbered globally so one launch covers the whole batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fvh {
+
+constexpr int kHeaderBytes = 32;
+constexpr int kDirectoryEntryBytes = 32;
+constexpr int kShardSymbols = 4096;
+constexpr int kBlockSymbols = 128;
+constexpr int kMaxRice = 16;
+constexpr int kModeZeroRun = 0;
+constexpr int kModeRiceBase = 1;
+constexpr int kModeBlockPack = 18;
+constexpr int kModeOrder0 = 19;
+constexpr int kEntropyParallelShards = 19;
+constexpr int kPredictorId = 6;
+constexpr int kRansAlphabet = 511;
+constexpr int kRansTableLog = 12;
+constexpr uint32_t kRansL = 1u << 23;
+
+// Worst-case record size of one full shard: block-pack bound
+// 32 * (1 + 128*17/8) = 8736 body bytes (>= order-0 bound 1024+28+4*1536)
+// plus the 3-byte record header.
+constexpr int kMaxRecordBytes = 8739;
+
+// Per-shard order-0 scratch layout (bytes).
+constexpr int kScratchTable = 0;        // 512 x u64 encode entries
+constexpr int kScratchSparse = 4096;    // sparse table bytes (<= 1024)
+constexpr int kScratchLanes = 5120;     // 4 lanes x kLaneBytes, filled from the end
+constexpr int kLaneBytes = 1536;        // proven bound for 1024 symbols
+constexpr int kScratchBytes = kScratchLanes + 4 * kLaneBytes;  // 11264
+
+// Frame-level status codes (first error wins).
+constexpr int kStatusSampleRange = 1;     // encoder: sample > max (encode.cu:160-162)
+constexpr int kStatusBodyTooLarge = 2;    // encoder: body > u16 (encode.cu:305-307)
+constexpr int kStatusTileTooLarge = 100;  // encoder: tile payload > u32 (encode.cu:898-899)
+constexpr int kStatusCapacity = 101;      // encoder: internal output bound violated
+constexpr int kStatusLaneOverflow = 102;  // encoder: rANS lane exceeded its bound
+constexpr int kStatusDirectory = 10;      // decoder: directory (decode.cu:131-186)
+constexpr int kStatusShardHeader = 11;    // decoder: shard walk (decode.cu:188-219)
+
+struct FrameDesc {
+  const uint16_t* plane[3];    // encode input planes (device)
+  uint16_t* out_plane[3];      // decode output planes (device, dense rows)
+  int64_t pitch[3];            // row pitch of plane[] in samples
+  int64_t plane_width[3];      // samples per row of each plane
+  int64_t plane_sample_base[3];// first folded sample of each plane within the frame
+  int64_t plane_shard_base[3]; // first shard of each plane within the frame
+  int64_t row_shards[3];       // shards in one full-height row of tiles
+  uint8_t* stream;             // encode: output stream; decode: input stream
+  int64_t stream_capacity;     // encode: output capacity; decode: stream size
+  int64_t folded_base;         // first folded element of the frame in the batch
+  int64_t total_samples;
+  int64_t width, height;
+  int32_t tile_w, tile_h;      // nominal luma tile size
+  int32_t ntw[3];              // nominal tile width per plane
+  int32_t cols[3];             // tile columns per plane
+  int32_t rows;                // tile rows (shared by all planes)
+  int32_t plane_count;
+  int32_t plane_tile_begin[4]; // local tile index where each plane starts; [3] = tile_count
+  int32_t tile_begin, tile_count;   // global tile range
+  int32_t shard_begin, shard_count; // global shard range
+  int32_t layout, bit_depth, quality, max_sample;
+  uint32_t fps_num, fps_den;
+  int32_t step0;               // quantization_step (encode.cu:71-85)
+  int32_t step_refined;        // refined_gray16_step (encode.cu:87-89)
+  int32_t force_order0;        // encode.cu:778
+  int32_t heuristic;           // 0 none, 1 gray10 MED candidate, 2 gray16 refinement candidate
+  int32_t use_med;             // decode: predictor choice
+  int32_t step;                // decode: step from the header
+  int32_t header_flags;        // decode: header byte 7 flag bits
+  int32_t pad;
+};
+
+// Per-frame state written on the device during a call.
+struct FrameState {
+  int32_t status;          // first error code (0 = ok)
+  int32_t step;            // active quantiser step of the current pass
+  int32_t use_med;         // active predictor of the current pass
+  int32_t active;          // current pass runs for this frame
+  int64_t payload;         // sum of record bytes of the last pass
+  int64_t baseline_payload;
+  int64_t stream_length;
+  int32_t gray_med;
+  int32_t refined;
+  int32_t baseline_step;
+  int32_t direct;
+  int32_t stage;           // heuristic stage machine
+  int32_t pad;
+};
+
+struct TileDesc {
+  int64_t folded_base;     // batch-global folded offset
+  int64_t src_offset;      // y*pitch + x within its plane
+  int64_t dst_offset;      // y*plane_width + x within its dense plane
+  int32_t frame, plane;
+  int32_t x, y, width, height;
+  int32_t first_shard, shard_count;   // global shard range
+};
+
+struct ShardDesc {
+  int32_t tile;            // global tile index
+  int32_t local;           // shard index within the tile
+};
+
+// Encoder per-shard analysis record.
+struct ShardRec {
+  uint8_t mode;            // chosen mode
+  uint8_t k;               // Rice parameter of the best-of-three analysis
+  uint8_t supported;       // order-0 feasible
+  uint8_t best3_mode;
+  uint16_t sparse_len;     // order-0 sparse table bytes
+  uint16_t distinct;
+  uint32_t body;           // chosen body bytes
+  uint32_t best3_body;     // best of zero-run / Rice / block-pack
+  uint16_t lane[4];        // chosen lane bytes (Rice or order-0)
+  uint16_t best3_lane[4];
+  uint32_t offset;         // record offset within its tile payload
+  uint32_t states[4];      // order-0 final states
+};
+
+// Decoder per-shard metadata (decode.cu:209-213).
+struct DShard {
+  int64_t body_offset;     // byte offset of the body within the stream
+  int64_t folded_offset;   // batch-global folded offset
+  int32_t frame;
+  int32_t mode;
+  int32_t body_length;
+  int32_t count;
+};
+
+struct Results {           // per frame, read back by the host
+  int64_t stream_length;
+  int32_t status;
+  int32_t flags;           // bit0 gray_med, bit1 refined
+};
+
+}  // namespace fvh

@@ -1,0 +1,1169 @@
+// fvh_encode.hip -- MI355X encoder kernels.
+//
+// Pipeline (all device-side, one launch per stage for the whole batch, no host
+// synchronisation inside a call):
+//   k_state_init   per-frame pass state
+//   k_wavefront    closed-loop predictor + quantiser, one wave64 per tile
+//   k_lossless     step-1 predictor (reconstruction == source: fully parallel)
+//   k_generic      wide-tile fallback (antidiagonal, recon in global memory)
+//   k_analyze      per-shard order-0 histogram/normalisation/sparse table and
+//                  exact zero-run / Rice / block-pack costs (wave per shard)
+//   k_rans         order-0 rANS lanes, 16 shards per wave, final mode choice
+//   k_tile_scan    record offsets inside each tile, tile payload lengths
+//   k_frame_finish tile offsets, frame payload, header + directory
+//   k_decide       gray10 MED / gray16 refinement decisions between passes
+//   k_emit         record assembly in LDS, coalesced copy to the stream
+#include "fvh_device.h"
+#include "fvh_kernels.h"
+#include "fvh_timing.h"
+
+namespace fvh {
+
+// ------------------------------------------------------------ frame upload
+
+constexpr int kUploadChunk = 8;
+struct FrameChunk {
+  FrameDesc f[kUploadChunk];
+  int n;
+  int first;
+};
+
+__global__ void k_upload(FrameChunk chunk, FrameDesc* __restrict__ dst) {
+  const int i = threadIdx.x;
+  if (i < chunk.n) dst[chunk.first + i] = chunk.f[i];
+}
+
+void launch_upload_frames(const FrameDesc* host_frames, int frame_count, FrameDesc* device_frames,
+                          hipStream_t stream) {
+  for (int first = 0; first < frame_count; first += kUploadChunk) {
+    FrameChunk chunk;
+    chunk.n = min(kUploadChunk, frame_count - first);
+    chunk.first = first;
+    for (int i = 0; i < chunk.n; ++i) chunk.f[i] = host_frames[first + i];
+    hipLaunchKernelGGL(k_upload, dim3(1), dim3(64), 0, stream, chunk, device_frames);
+  }
+}
+
+// ----------------------------------------------------------- pass control
+
+__global__ void k_state_init(const FrameDesc* __restrict__ frames, FrameState* __restrict__ states,
+                             int frame_count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= frame_count) return;
+  FrameState s = {};
+  s.step = frames[i].step0;
+  s.use_med = frames[i].layout != 0;  // first pass: MED for colour, gradient for gray (encode.cu:812)
+  s.active = 1;
+  states[i] = s;
+}
+
+// Between passes: the multi-pass heuristics of encode.cu:812-847.
+//   stage 1 (after the first pass): gray10 lossy re-runs with MED when the
+//     payload exceeds 0.3 B/sample; gray16 lossy takes the refined step when
+//     the payload is < 0.1 or > 0.5 B/sample (direct) or in [0.1, 0.2) (probe).
+//   stage 2 (after the refined pass): a probed gray16 frame keeps the refined
+//     step only below 0.1 B/sample, otherwise re-runs the baseline step.
+//   stage 3: every frame active again for assembly and emission.
+__global__ void k_decide(const FrameDesc* __restrict__ frames, FrameState* __restrict__ states,
+                         int frame_count, int stage) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= frame_count) return;
+  const FrameDesc& f = frames[i];
+  FrameState& s = states[i];
+  const int64_t samples = f.total_samples;
+  if (stage == 1) {
+    const int was_active = s.active;
+    s.active = 0;
+    if (!was_active || s.status != 0) return;
+    s.baseline_payload = s.payload;
+    if (f.heuristic == 1) {
+      s.gray_med = s.payload * 10 > samples * 3;
+      if (s.gray_med) {
+        s.use_med = 1;
+        s.active = 1;
+      }
+    } else if (f.heuristic == 2) {
+      const int64_t base = s.payload;
+      s.direct = base < samples / 10 || base > samples / 2;
+      const int probe = !s.direct && base < samples / 5;
+      if (s.direct || probe) {
+        s.baseline_step = s.step;
+        s.step = f.step_refined;
+        s.use_med = 0;
+        s.active = 1;
+        s.stage = 2;
+      }
+    }
+  } else if (stage == 2) {
+    const int was_active = s.active;
+    s.active = 0;
+    if (!was_active || s.status != 0 || f.heuristic != 2 || s.stage != 2) return;
+    s.refined = s.direct || s.payload < samples / 10;
+    if (!s.refined) {
+      s.step = s.baseline_step;
+      s.active = 1;
+    }
+  } else {
+    s.active = 1;
+  }
+}
+
+// ---------------------------------------------------- predictor + quantiser
+
+// One wave64 per tile.  Lane l owns rows l, 64+l, 128+l, ... of the tile and
+// runs one step behind lane l-1, so at every step lane l needs the sample
+// "above" that lane l-1 reconstructed at the previous step: a single wave
+// shuffle, no barrier.  Consecutive 64-row bands are pipelined back to back;
+// lane 0 takes the last row of the previous band from an LDS row written by
+// lane 63 (W-63 steps earlier).  Total steps per tile: bands*max(W,64)+W+63
+// instead of the reference's W+H-1 barrier-separated antidiagonals
+// (encode.cu:150-192, decode.cu:572-605), with no reconstruction buffer in
+// global memory.
+//
+// kEncode: source -> residual (zigzag, tile order) with closed-loop
+//          reconstruction (encode.cu:159-189).
+// decode:  residual -> reconstruction written to the output plane
+//          (decode.cu:581-602).
+template <bool kEncode, typename FT>
+__global__ __launch_bounds__(256) void k_wavefront(const FrameDesc* __restrict__ frames,
+                                                   FrameState* __restrict__ states,
+                                                   const TileDesc* __restrict__ tiles,
+                                                   int tile_total, FT* __restrict__ folded) {
+  __shared__ uint16_t ring_all[4][kRingMax];
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int t = blockIdx.x * 4 + wave;
+  if (t >= tile_total) return;
+  const TileDesc td = tiles[t];
+  FrameState* st = &states[td.frame];
+  const FrameDesc& f = frames[td.frame];
+  int step, med;
+  if (kEncode) {
+    if (!st->active) return;
+    step = st->step;
+    med = st->use_med;
+    if (step == 1) return;  // lossless: k_lossless
+  } else {
+    if (st->status != 0) return;
+    step = f.step;
+    med = f.use_med;
+  }
+  const int W = td.width, H = td.height;
+  if (W > kRingMax) return;  // k_generic
+  const int max_sample = f.max_sample;
+  const uint32_t half = (uint32_t)(step >> 1);
+  const uint32_t magic = step > 1 ? quant_magic(step) : 0u;
+  uint16_t* ring = ring_all[wave];
+  const int nb = (H + 63) >> 6;
+  const int P = W > 64 ? W : 64;
+  const int total = (nb - 1) * P + W + 63;
+  FT* fold = folded + td.folded_base;
+  const uint16_t* src = nullptr;
+  int64_t src_pitch = 0;
+  uint16_t* dst = nullptr;
+  int64_t dst_pitch = 0;
+  if (kEncode) {
+    src = f.plane[td.plane] + td.src_offset;
+    src_pitch = f.pitch[td.plane];
+  } else {
+    dst = f.out_plane[td.plane] + td.dst_offset;
+    dst_pitch = f.plane_width[td.plane];
+  }
+  int b = 0, x = -lane;
+  int L = 0, UL = 0, prev = 0, bad = 0;
+  for (int s = 0; s < total; ++s) {
+    const int nbr = shfl_up(prev, 1);
+    const bool in_tile = x >= 0 && x < W && b < nb;
+    const int y = (b << 6) + lane;
+    int ringv = 0;
+    if (lane == 0 && b > 0 && in_tile) ringv = ring[x];
+    if (in_tile && y < H) {
+      const int above = y == 0 ? 0 : (lane == 0 ? ringv : nbr);
+      const int left = x > 0 ? L : 0;
+      const int ul = (x > 0 && y > 0) ? UL : 0;
+      const int grad = left + above - ul;
+      const int pred = med ? median3(left, above, grad) : clampi(grad, 0, max_sample);
+      int rec;
+      if (kEncode) {
+        const int sample = src[(int64_t)y * src_pitch + x];
+        bad |= sample > max_sample;
+        const int r = sample - pred;
+        const uint32_t mag = __umulhi((uint32_t)abs(r) + half, magic);
+        const int q = r < 0 ? -(int)mag : (int)mag;
+        rec = clampi(pred + q * step, 0, max_sample);
+        fold[(int64_t)y * W + x] = (FT)(((uint32_t)q << 1) ^ (uint32_t)(q >> 31));
+      } else {
+        const uint32_t fv = fold[(int64_t)y * W + x];
+        const int q = (fv & 1u) ? -(int)(fv >> 1) - 1 : (int)(fv >> 1);
+        rec = clampi(pred + q * step, 0, max_sample);
+        dst[(int64_t)y * dst_pitch + x] = (uint16_t)rec;
+      }
+      if (lane == 63 && b + 1 < nb) ring[x] = (uint16_t)rec;
+      L = rec;
+      UL = above;
+      prev = rec;
+    }
+    if (++x == P) {
+      x = 0;
+      ++b;
+    }
+  }
+  if (kEncode && ballot(bad != 0) != 0 && lane == 0) set_status(st, kStatusSampleRange);
+}
+
+// Step 1: the reconstruction equals the source for in-range samples, so each
+// residual depends only on source neighbours and every sample is independent.
+template <typename FT>
+__global__ __launch_bounds__(256) void k_lossless(const FrameDesc* __restrict__ frames,
+                                                  FrameState* __restrict__ states,
+                                                  const TileDesc* __restrict__ tiles,
+                                                  FT* __restrict__ folded) {
+  const int t = blockIdx.x;
+  const TileDesc td = tiles[t];
+  FrameState* st = &states[td.frame];
+  if (!st->active || st->step != 1) return;
+  const FrameDesc& f = frames[td.frame];
+  const int med = st->use_med;
+  const int max_sample = f.max_sample;
+  const uint16_t* src = f.plane[td.plane] + td.src_offset;
+  const int64_t pitch = f.pitch[td.plane];
+  FT* fold = folded + td.folded_base;
+  const int W = td.width, H = td.height;
+  int bad = 0;
+  for (int y = threadIdx.x >> 6; y < H; y += 4) {
+    const uint16_t* row = src + (int64_t)y * pitch;
+    for (int x = lane_id(); x < W; x += 64) {
+      const int sample = row[x];
+      const int left = x > 0 ? row[x - 1] : 0;
+      const int above = y > 0 ? row[x - pitch] : 0;
+      const int ul = (x > 0 && y > 0) ? row[x - pitch - 1] : 0;
+      const int grad = left + above - ul;
+      const int pred = med ? median3(left, above, grad) : clampi(grad, 0, max_sample);
+      const int q = sample - pred;
+      bad |= sample > max_sample;
+      fold[(int64_t)y * W + x] = (FT)(((uint32_t)q << 1) ^ (uint32_t)(q >> 31));
+    }
+  }
+  if (ballot(bad != 0) != 0 && lane_id() == 0) set_status(st, kStatusSampleRange);
+}
+
+// Tiles wider than kRingMax: the reference's antidiagonal schedule with the
+// reconstruction kept in global memory (tile order for the encoder, the
+// output plane for the decoder).
+template <bool kEncode, typename FT>
+__global__ __launch_bounds__(256) void k_generic(const FrameDesc* __restrict__ frames,
+                                                 FrameState* __restrict__ states,
+                                                 const TileDesc* __restrict__ tiles,
+                                                 FT* __restrict__ folded,
+                                                 uint16_t* __restrict__ recon) {
+  const int t = blockIdx.x;
+  const TileDesc td = tiles[t];
+  FrameState* st = &states[td.frame];
+  const FrameDesc& f = frames[td.frame];
+  const int64_t W = td.width, H = td.height;
+  if (W <= kRingMax) return;
+  int step, med;
+  if (kEncode) {
+    if (!st->active) return;
+    step = st->step;
+    med = st->use_med;
+  } else {
+    if (st->status != 0) return;
+    step = f.step;
+    med = f.use_med;
+  }
+  const int max_sample = f.max_sample;
+  const uint32_t half = (uint32_t)(step >> 1);
+  const uint32_t magic = step > 1 ? quant_magic(step) : 0u;
+  FT* fold = folded + td.folded_base;
+  uint16_t* rc;
+  int64_t rc_pitch;
+  const uint16_t* src = nullptr;
+  int64_t src_pitch = 0;
+  if (kEncode) {
+    rc = recon + td.folded_base;
+    rc_pitch = W;
+    src = f.plane[td.plane] + td.src_offset;
+    src_pitch = f.pitch[td.plane];
+  } else {
+    rc = f.out_plane[td.plane] + td.dst_offset;
+    rc_pitch = f.plane_width[td.plane];
+  }
+  int bad = 0;
+  for (int64_t d = 0; d < W + H - 1; ++d) {
+    const int64_t y0 = d - W + 1 > 0 ? d - W + 1 : 0;
+    const int64_t y1 = d < H - 1 ? d : H - 1;
+    for (int64_t item = threadIdx.x; item <= y1 - y0; item += blockDim.x) {
+      const int64_t y = y0 + item;
+      const int64_t x = d - y;
+      uint16_t* p = rc + y * rc_pitch + x;
+      const int left = x > 0 ? p[-1] : 0;
+      const int above = y > 0 ? p[-rc_pitch] : 0;
+      const int ul = (x > 0 && y > 0) ? p[-rc_pitch - 1] : 0;
+      const int grad = left + above - ul;
+      const int pred = med ? median3(left, above, grad) : clampi(grad, 0, max_sample);
+      if (kEncode) {
+        const int sample = src[y * src_pitch + x];
+        bad |= sample > max_sample;
+        const int r = sample - pred;
+        int q;
+        if (step == 1) {
+          q = r;
+        } else {
+          const uint32_t mag = __umulhi((uint32_t)abs(r) + half, magic);
+          q = r < 0 ? -(int)mag : (int)mag;
+        }
+        *p = (uint16_t)clampi(pred + q * step, 0, max_sample);
+        fold[y * W + x] = (FT)(((uint32_t)q << 1) ^ (uint32_t)(q >> 31));
+      } else {
+        const uint32_t fv = fold[y * W + x];
+        const int q = (fv & 1u) ? -(int)(fv >> 1) - 1 : (int)(fv >> 1);
+        *p = (uint16_t)clampi(pred + q * step, 0, max_sample);
+      }
+    }
+    __syncthreads();
+  }
+  if (kEncode && bad) set_status(st, kStatusSampleRange);
+}
+
+// ------------------------------------------------------------- analysis
+
+// Clamp function a -> min(max(a + d, lo), hi).  The order-0 normalisation
+// end_i = min(max(E_i, end_{i-1} + 1), 4096 - remaining_i) (encode.cu:370-390)
+// is a chain of such functions; clamp functions compose into clamp
+// functions, so the chain is evaluated as a wave-wide prefix scan.
+struct ClampFn {
+  int d, lo, hi;
+};
+
+__device__ __forceinline__ ClampFn clamp_identity() { return ClampFn{0, -(1 << 30), 1 << 30}; }
+
+// first g1, then g2
+__device__ __forceinline__ ClampFn clamp_compose(ClampFn g1, ClampFn g2) {
+  ClampFn r;
+  r.d = g1.d + g2.d;
+  r.lo = clampi(g1.lo + g2.d, g2.lo, g2.hi);
+  r.hi = clampi(g1.hi + g2.d, g2.lo, g2.hi);
+  return r;
+}
+
+__device__ __forceinline__ int clamp_apply(ClampFn g, int a) { return min(max(a + g.d, g.lo), g.hi); }
+
+__device__ __forceinline__ ClampFn norm_fn(uint32_t prefix, int count, int remaining) {
+  if (remaining == 0) return ClampFn{0, 4096, 4096};
+  const int e = count == kShardSymbols ? (int)prefix
+                                        : (int)(((uint64_t)prefix << kRansTableLog) / (uint64_t)count);
+  const int u = 4096 - remaining;
+  if (e > u) return ClampFn{0, u, u};
+  return ClampFn{1, e, u};
+}
+
+// rANS encoder entry of one symbol: freq (13 bits) | cum << 13 (12 bits) |
+// c << 25 (4 bits) | m << 32, with c = ceil(log2 freq) and
+// m = ceil(2^(31+c) / freq): floor(x / freq) = mul_hi(x, m) >> (c-1) for every
+// x < 2^31 (freq >= 2; freq == 1 divides trivially).
+__device__ __forceinline__ uint64_t rans_entry(uint32_t freq, uint32_t cum) {
+  uint32_t c = 0;
+  uint32_t m = 0;
+  if (freq >= 2) {
+    c = 32 - __builtin_clz(freq - 1);
+    m = (uint32_t)(((1ull << (31 + c)) + freq - 1) / freq);
+  }
+  return (uint64_t)freq | ((uint64_t)cum << 13) | ((uint64_t)c << 25) | ((uint64_t)m << 32);
+}
+
+__device__ __forceinline__ int write_varint_bytes(uint8_t* out, uint32_t value) {
+  int n = 0;
+  do {
+    uint8_t byte = (uint8_t)(value & 0x7f);
+    value >>= 7;
+    if (value != 0) byte |= 0x80;
+    out[n++] = byte;
+  } while (value != 0);
+  return n;
+}
+
+// One wave64 per shard; lane l holds symbols l, 64+l, ..., so symbol i sits
+// in lane i%64 and its Rice lane i%4 equals l%4.  Everything the reference
+// computes with shared atomics, a tree reduction and one serial thread
+// (encode.cu:195-431) becomes ballots, popcounts and wave scans here.
+template <typename FT>
+__global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ frames,
+                                                 FrameState* __restrict__ states,
+                                                 const TileDesc* __restrict__ tiles,
+                                                 const ShardDesc* __restrict__ shards,
+                                                 int shard_total, const FT* __restrict__ folded,
+                                                 ShardRec* __restrict__ recs,
+                                                 uint8_t* __restrict__ scratch) {
+  __shared__ uint32_t hist_all[4][512];
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int s = blockIdx.x * 4 + wave;
+  if (s >= shard_total) return;
+  const ShardDesc sd = shards[s];
+  const TileDesc td = tiles[sd.tile];
+  FrameState* st = &states[td.frame];
+  if (!st->active) return;
+  const int force = frames[td.frame].force_order0;
+  const int64_t local0 = (int64_t)sd.local * kShardSymbols;
+  const int count = (int)min((int64_t)kShardSymbols, (int64_t)td.width * td.height - local0);
+  const FT* v = folded + td.folded_base + local0;
+  uint32_t* hist = hist_all[wave];
+  for (int i = lane; i < 512; i += 64) hist[i] = 0;
+  __builtin_amdgcn_wave_barrier();
+
+  uint32_t val[64];
+  uint32_t lmax = 0;
+  int unsup = 0;
+  int zeros = 0, starts = 0, over64 = 0, over8192 = 0;
+  uint64_t prevZ = 0, myZ = 0;
+#pragma unroll
+  for (int j = 0; j < 64; ++j) {
+    const int i = (j << 6) + lane;
+    const bool valid = i < count;
+    const uint32_t x = valid ? (uint32_t)v[i] : 0u;
+    val[j] = x;
+    lmax = max(lmax, x);
+    const uint64_t Z = ballot(valid && x == 0);
+    zeros += popc64(Z);
+    starts += popc64(Z & ~((Z << 1) | (prevZ >> 63)));
+    prevZ = Z;
+    myZ = lane == j ? Z : myZ;
+    over64 += popc64(ballot(valid && x > 64));
+    over8192 += popc64(ballot(valid && x > 8192));
+    if (valid) {
+      if (x < (uint32_t)kRansAlphabet) atomicAdd(&hist[x], 1u);
+      else unsup = 1;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t maxv = wave_max(lmax);
+  bool supported = ballot(unsup != 0) == 0;
+
+  ShardRec rec = {};
+  rec.best3_body = 0xffffffffu;
+  uint8_t* sbase = scratch + (int64_t)s * kScratchBytes;
+
+  // ---------------- order-0 table (encode.cu:337-392)
+  int distinct = 0;
+  if (supported) {
+    uint32_t c[8];
+    uint32_t csum = 0;
+    int npres = 0, last = -1;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      c[k] = hist[lane * 8 + k];
+      csum += c[k];
+      if (c[k]) {
+        ++npres;
+        last = lane * 8 + k;
+      }
+    }
+    const int rank_incl = wave_incl_scan(npres);
+    const uint32_t cnt_incl = wave_incl_scan(csum);
+    distinct = shfl(rank_incl, 63);
+    if (distinct > 255) supported = false;
+    if (supported) {
+      const int rank0 = rank_incl - npres;
+      // compose this lane's normalisation steps
+      ClampFn g = clamp_identity();
+      {
+        uint32_t p = cnt_incl - csum;
+        int rank = rank0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          if (c[k]) {
+            p += c[k];
+            g = clamp_compose(g, norm_fn(p, count, distinct - rank - 1));
+            ++rank;
+          }
+        }
+      }
+#pragma unroll
+      for (int dlt = 1; dlt < 64; dlt <<= 1) {
+        ClampFn o;
+        o.d = shfl_up(g.d, dlt);
+        o.lo = shfl_up(g.lo, dlt);
+        o.hi = shfl_up(g.hi, dlt);
+        if (lane >= dlt) g = clamp_compose(o, g);
+      }
+      ClampFn e;
+      e.d = shfl_up(g.d, 1);
+      e.lo = shfl_up(g.lo, 1);
+      e.hi = shfl_up(g.hi, 1);
+      if (lane == 0) e = clamp_identity();
+      int assigned = clamp_apply(e, 0);
+      // previous present value before this lane (0 if none)
+      int prev_val = last;
+#pragma unroll
+      for (int dlt = 1; dlt < 64; dlt <<= 1) {
+        const int o = shfl_up(prev_val, dlt);
+        if (lane >= dlt) prev_val = max(prev_val, o);
+      }
+      int before = shfl_up(prev_val, 1);
+      if (lane == 0 || before < 0) before = 0;
+      // frequencies, encoder entries and sparse-table entry sizes
+      uint64_t* tab = (uint64_t*)(sbase + kScratchTable);
+      uint32_t freqs[8];
+      int lens = 0;
+      {
+        uint32_t p = cnt_incl - csum;
+        int rank = rank0;
+        int pv = before;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          freqs[k] = 0;
+          if (c[k]) {
+            p += c[k];
+            const int rem = distinct - rank - 1;
+            const int end = clamp_apply(norm_fn(p, count, rem), assigned);
+            freqs[k] = (uint32_t)(end - assigned);
+            tab[lane * 8 + k] = rans_entry(freqs[k], (uint32_t)assigned);
+            lens += varint_len((uint32_t)(lane * 8 + k - pv)) + (rem ? varint_len(freqs[k]) : 0);
+            assigned = end;
+            pv = lane * 8 + k;
+            ++rank;
+          }
+        }
+      }
+      const int len_incl = wave_incl_scan(lens);
+      const int hdr = 1 + varint_len((uint32_t)distinct);
+      uint8_t* sparse = sbase + kScratchSparse;
+      if (lane == 0) {
+        sparse[0] = (uint8_t)kRansTableLog;
+        write_varint_bytes(sparse + 1, (uint32_t)distinct);
+      }
+      {
+        int pos = hdr + len_incl - lens;
+        int rank = rank0;
+        int pv = before;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          if (c[k]) {
+            const int rem = distinct - rank - 1;
+            pos += write_varint_bytes(sparse + pos, (uint32_t)(lane * 8 + k - pv));
+            if (rem) pos += write_varint_bytes(sparse + pos, freqs[k]);
+            pv = lane * 8 + k;
+            ++rank;
+          }
+        }
+      }
+      rec.sparse_len = (uint16_t)(hdr + shfl(len_incl, 63));
+      rec.distinct = (uint16_t)distinct;
+      rec.supported = 1;
+    }
+  }
+
+  // ---------------- best of zero-run / Rice / block-pack (encode.cu:226-307)
+  if (!force || !supported) {
+    // long zero runs (> 64 symbols cost a 2-byte token): runs whose start p
+    // has zeros at p..p+64, found from this lane's 64-bit zero word and the next
+    uint64_t zp = shfl_up(myZ, 1);
+    uint64_t zn = __shfl_down(myZ, 1, 64);
+    if (lane == 0) zp = 0;
+    if (lane == 63) zn = 0;
+    const uint64_t st_bits = myZ & ~((myZ << 1) | (zp >> 63));
+    const uint64_t nz = ~myZ, nzn = ~zn;
+    const int lo = nz == 0 ? 0 : 64 - __clzll((long long)nz);
+    const int hi = nzn == 0 ? 64 : __ffsll((long long)nzn) - 1;
+    uint64_t m = 0;
+    if (lo < hi) {
+      const uint64_t upper = hi == 64 ? ~0ull : ((1ull << hi) - 1);
+      m = upper & ~((1ull << lo) - 1);
+    }
+    const int longs = wave_sum(popc64(st_bits & m));
+    const int64_t zero_bytes = (int64_t)(count - zeros) + over64 + over8192 + starts + longs;
+
+    int64_t block_bytes = 0;
+#pragma unroll
+    for (int b = 0; b < 32; ++b) {
+      const uint32_t o = wave_or(val[2 * b] | val[2 * b + 1]);
+      const int n = clampi(count - b * kBlockSymbols, 0, kBlockSymbols);
+      if (n > 0) block_bytes += 1 + (n * bit_length(o) + 7) / 8;
+    }
+
+    const int lane_count = count < 4 ? count : 4;
+    const int bl = bit_length(maxv);
+    int best_k = 0;
+    int64_t best_bytes = INT64_MAX;
+    uint32_t best_lane[4] = {0, 0, 0, 0};
+    for (int k = 0; k <= kMaxRice; ++k) {
+      uint32_t sum = 0;
+      if (k < bl) {
+#pragma unroll
+        for (int j = 0; j < 64; ++j) sum += val[j] >> k;
+        sum += shfl_xor(sum, 4);
+        sum += shfl_xor(sum, 8);
+        sum += shfl_xor(sum, 16);
+        sum += shfl_xor(sum, 32);
+      }
+      int64_t bytes = 0;
+      uint32_t lb[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t cls = shfl(sum, r);
+        const uint32_t n_r = r < lane_count ? (uint32_t)((count - r + 3) / 4) : 0u;
+        const uint64_t bits = (uint64_t)cls + (uint64_t)n_r * (uint64_t)(1 + k);
+        lb[r] = r < lane_count ? (uint32_t)((bits + 7) / 8) : 0u;
+        bytes += lb[r];
+      }
+      if (bytes < best_bytes) {
+        best_bytes = bytes;
+        best_k = k;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) best_lane[r] = lb[r];
+      }
+    }
+    const int64_t rice_bytes = (int64_t)(lane_count - 1) * 4 + best_bytes;
+    int mode;
+    int64_t body;
+    if (block_bytes < zero_bytes && block_bytes < rice_bytes) {
+      mode = kModeBlockPack;
+      body = block_bytes;
+    } else if (zero_bytes <= rice_bytes) {
+      mode = kModeZeroRun;
+      body = zero_bytes;
+    } else {
+      mode = kModeRiceBase + best_k;
+      body = rice_bytes;
+    }
+    if (body > 65535 && lane == 0) set_status(st, kStatusBodyTooLarge);
+    rec.best3_mode = (uint8_t)mode;
+    rec.mode = (uint8_t)mode;
+    rec.k = (uint8_t)best_k;
+    rec.best3_body = (uint32_t)min(body, (int64_t)0xffffffff);
+    rec.body = rec.best3_body;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      rec.best3_lane[r] = (uint16_t)min(best_lane[r], 65535u);
+      rec.lane[r] = rec.best3_lane[r];
+    }
+  } else {
+    rec.mode = kModeOrder0;
+  }
+  if (lane == 0) recs[s] = rec;
+}
+
+// Order-0 rANS encode (encode.cu:398-418): one thread per rANS lane, four
+// consecutive threads per shard, 16 shards per wave.  Division by the symbol
+// frequency uses the per-symbol reciprocal from k_analyze.  Emitted bytes are
+// stored backwards from the end of the lane's scratch area, so the area ends
+// with the lane's bytes already in decode order.  The last step decides the
+// shard's mode: order-0 when forced or strictly smaller than the best of
+// three (encode.cu:423-429).
+template <typename FT>
+__global__ __launch_bounds__(256) void k_rans(const FrameDesc* __restrict__ frames,
+                                              FrameState* __restrict__ states,
+                                              const TileDesc* __restrict__ tiles,
+                                              const ShardDesc* __restrict__ shards,
+                                              int shard_total, const FT* __restrict__ folded,
+                                              ShardRec* __restrict__ recs,
+                                              uint8_t* __restrict__ scratch) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gid >> 2;
+  const int r = gid & 3;
+  if (s >= shard_total) return;
+  const ShardDesc sd = shards[s];
+  const TileDesc td = tiles[sd.tile];
+  FrameState* st = &states[td.frame];
+  const bool act = st->active && recs[s].supported;
+  int nbytes = 0;
+  if (act) {
+    const int64_t local0 = (int64_t)sd.local * kShardSymbols;
+    const int count = (int)min((int64_t)kShardSymbols, (int64_t)td.width * td.height - local0);
+    const FT* v = folded + td.folded_base + local0;
+    uint8_t* sbase = scratch + (int64_t)s * kScratchBytes;
+    const uint64_t* tab = (const uint64_t*)(sbase + kScratchTable);
+    uint8_t* lane_end = sbase + kScratchLanes + (r + 1) * kLaneBytes;
+    uint32_t x = kRansL;
+    for (int i = count - 1 - ((count - 1 - r) & 3); i >= 0; i -= 4) {
+      const uint64_t e = tab[v[i]];
+      const uint32_t lo = (uint32_t)e;
+      const uint32_t freq = lo & 0x1fffu;
+      const uint32_t cum = (lo >> 13) & 0xfffu;
+      const uint32_t c = (lo >> 25) & 0xfu;
+      const uint32_t m = (uint32_t)(e >> 32);
+      const uint32_t lim = freq << 19;
+      if (x >= lim) {
+        if (nbytes < kLaneBytes) lane_end[-1 - nbytes] = (uint8_t)x;
+        ++nbytes;
+        x >>= 8;
+        if (x >= lim) {
+          if (nbytes < kLaneBytes) lane_end[-1 - nbytes] = (uint8_t)x;
+          ++nbytes;
+          x >>= 8;
+        }
+      }
+      const uint32_t q = c ? (__umulhi(x, m) >> (c - 1)) : x;
+      x += q * (4096u - freq) + cum;
+    }
+    if (nbytes > kLaneBytes) set_status(st, kStatusLaneOverflow);
+    recs[s].states[r] = x;
+  }
+  const int base = lane_id() & ~3;
+  const int b0 = shfl(nbytes, base);
+  const int b1 = shfl(nbytes, base + 1);
+  const int b2 = shfl(nbytes, base + 2);
+  const int b3 = shfl(nbytes, base + 3);
+  if (act && r == 0) {
+    ShardRec& rec = recs[s];
+    const uint32_t body0 = (uint32_t)rec.sparse_len + 28u + (uint32_t)(b0 + b1 + b2 + b3);
+    if (frames[td.frame].force_order0 || body0 < rec.best3_body) {
+      rec.mode = kModeOrder0;
+      rec.body = body0;
+      rec.lane[0] = (uint16_t)b0;
+      rec.lane[1] = (uint16_t)b1;
+      rec.lane[2] = (uint16_t)b2;
+      rec.lane[3] = (uint16_t)b3;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- assembly
+
+// Record offsets inside each tile (records are [mode u8][body u16][body]).
+__global__ __launch_bounds__(256) void k_tile_scan(FrameState* __restrict__ states,
+                                                   const TileDesc* __restrict__ tiles,
+                                                   int tile_total, ShardRec* __restrict__ recs,
+                                                   int64_t* __restrict__ tile_len, int all) {
+  const int lane = lane_id();
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= tile_total) return;
+  const TileDesc td = tiles[t];
+  if (!all && !states[td.frame].active) return;
+  int64_t carry = 0;
+  for (int c0 = 0; c0 < td.shard_count; c0 += 64) {
+    const int i = c0 + lane;
+    const int64_t rb = i < td.shard_count ? (int64_t)recs[td.first_shard + i].body + 3 : 0;
+    const int64_t incl = wave_incl_scan(rb);
+    if (i < td.shard_count) recs[td.first_shard + i].offset = (uint32_t)(carry + incl - rb);
+    carry += shfl(incl, 63);
+  }
+  if (lane == 0) tile_len[t] = carry;
+}
+
+__device__ __forceinline__ void put_words(uint8_t* p, const uint32_t* w, int n) {
+  uint32_t* q = (uint32_t*)p;
+  for (int i = 0; i < n; ++i) q[i] = w[i];
+}
+
+// One workgroup per frame: exclusive scan of tile payload lengths into tile
+// offsets, the frame payload (the heuristics' input), and on the final pass
+// the 32-byte header and 32-byte directory entries (encode.cu:858-904).
+__global__ __launch_bounds__(256) void k_frame_finish(const FrameDesc* __restrict__ frames,
+                                                      FrameState* __restrict__ states,
+                                                      const TileDesc* __restrict__ tiles,
+                                                      const int64_t* __restrict__ tile_len,
+                                                      int64_t* __restrict__ tile_off,
+                                                      Results* __restrict__ results, int final) {
+  __shared__ int64_t part[256];
+  __shared__ int too_large;
+  const FrameDesc& f = frames[blockIdx.x];
+  FrameState* st = &states[blockIdx.x];
+  if (!final && !st->active) return;
+  const int tid = threadIdx.x;
+  if (tid == 0) too_large = 0;
+  const int T = f.tile_count;
+  const int64_t payload_start = kHeaderBytes + (int64_t)T * kDirectoryEntryBytes;
+  int64_t running = 0;
+  for (int c0 = 0; c0 < T; c0 += 256) {
+    const int i = c0 + tid;
+    const int64_t len = i < T ? tile_len[f.tile_begin + i] : 0;
+    __syncthreads();
+    part[tid] = len;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+      const int64_t o = tid >= d ? part[tid - d] : 0;
+      __syncthreads();
+      part[tid] += o;
+      __syncthreads();
+    }
+    const int64_t off = payload_start + running + part[tid] - len;
+    if (i < T) {
+      tile_off[f.tile_begin + i] = off;
+      if (final) {
+        if (len > 0xffffffffll) too_large = 1;
+        const TileDesc td = tiles[f.tile_begin + i];
+        uint32_t w[8];
+        w[0] = (uint32_t)td.plane | ((uint32_t)kEntropyParallelShards << 8) |
+               ((uint32_t)kPredictorId << 16);
+        w[1] = (uint32_t)td.x;
+        w[2] = (uint32_t)td.y;
+        w[3] = (uint32_t)td.width;
+        w[4] = (uint32_t)td.height;
+        w[5] = (uint32_t)(uint64_t)off;
+        w[6] = (uint32_t)((uint64_t)off >> 32);
+        w[7] = (uint32_t)len;
+        if (st->status == 0) put_words(f.stream + kHeaderBytes + (int64_t)i * kDirectoryEntryBytes, w, 8);
+      }
+    }
+    running += part[255];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    st->payload = running;
+    if (final) {
+      const int64_t length = payload_start + running;
+      if (too_large) set_status(st, kStatusTileTooLarge);
+      if (length > f.stream_capacity) set_status(st, kStatusCapacity);
+      const int flags = (st->gray_med ? 0x40 : 0) | (st->refined ? 0x80 : 0);
+      if (st->status == 0) {
+        uint32_t w[8];
+        w[0] = 'F' | ('V' << 8) | ('I' << 16) | ((uint32_t)'D' << 24);
+        w[1] = 1u | ((uint32_t)f.layout << 8) | ((uint32_t)f.quality << 16) |
+               ((uint32_t)((f.bit_depth - 8) | flags) << 24);
+        w[2] = (uint32_t)f.width;
+        w[3] = (uint32_t)f.height;
+        w[4] = (uint32_t)f.tile_w | ((uint32_t)f.tile_h << 16);
+        w[5] = f.fps_num;
+        w[6] = f.fps_den;
+        w[7] = (uint32_t)T;
+        put_words(f.stream, w, 8);
+      }
+      st->stream_length = length;
+      Results res;
+      res.stream_length = length;
+      res.status = st->status;
+      res.flags = (st->gray_med ? 1 : 0) | (st->refined ? 2 : 0);
+      results[blockIdx.x] = res;
+    }
+  }
+}
+
+// ------------------------------------------------------------- emission
+
+constexpr int kBufWords = (kMaxRecordBytes + 3) / 4 + 2;
+
+__device__ __forceinline__ void or_bits(uint32_t* buf, uint32_t pos, uint32_t pattern) {
+  const uint64_t pp = (uint64_t)pattern << (pos & 31);
+  const uint32_t w = pos >> 5;
+  if ((uint32_t)pp) atomicOr(&buf[w], (uint32_t)pp);
+  if ((uint32_t)(pp >> 32)) atomicOr(&buf[w + 1], (uint32_t)(pp >> 32));
+}
+
+// One wave64 per shard builds the record [mode][body u16][body] in LDS and
+// copies it to its offset; the reference instead sets bits with global
+// atomics into a zero-filled stream (encode.cu:458-474, 906).
+template <typename FT>
+__global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ frames,
+                                              const FrameState* __restrict__ states,
+                                              const TileDesc* __restrict__ tiles,
+                                              const ShardDesc* __restrict__ shards,
+                                              int shard_total, const FT* __restrict__ folded,
+                                              const ShardRec* __restrict__ recs,
+                                              const uint8_t* __restrict__ scratch,
+                                              const int64_t* __restrict__ tile_off) {
+  __shared__ uint32_t buf_all[4][kBufWords];
+  __shared__ uint64_t zm_all[4][65];
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int s = blockIdx.x * 4 + wave;
+  if (s >= shard_total) return;
+  const ShardDesc sd = shards[s];
+  const TileDesc td = tiles[sd.tile];
+  if (states[td.frame].status != 0) return;
+  const FrameDesc& f = frames[td.frame];
+  const ShardRec rec = recs[s];
+  const int64_t local0 = (int64_t)sd.local * kShardSymbols;
+  const int count = (int)min((int64_t)kShardSymbols, (int64_t)td.width * td.height - local0);
+  const FT* v = folded + td.folded_base + local0;
+  uint8_t* out = f.stream + tile_off[sd.tile] + rec.offset;
+  const int total = 3 + (int)rec.body;
+  uint32_t* buf = buf_all[wave];
+  uint8_t* bytes = (uint8_t*)buf;
+  uint8_t* body = bytes + 3;
+  const int words = (total + 3) >> 2;
+  for (int i = lane; i < words + 1; i += 64) buf[i] = 0;
+  __builtin_amdgcn_wave_barrier();
+  const int mode = rec.mode;
+
+  if (mode == kModeZeroRun) {
+    // zero-run tokens (encode.cu:514-536): a nonzero v is varint(2v-1), a
+    // maximal zero run of R symbols is varint(2(R-1)) at the run's start.
+    uint64_t* zm = zm_all[wave];
+    for (int j = 0; j < 64; ++j) {
+      const int i = (j << 6) + lane;
+      const uint64_t Z = ballot(i < count && v[i] == 0);
+      if (lane == 0) zm[j] = Z;
+    }
+    if (lane == 0) zm[64] = 0;
+    __builtin_amdgcn_wave_barrier();
+    int carry = 0;
+    uint64_t prevZ = 0;
+    for (int j = 0; j < 64 && (j << 6) < count; ++j) {
+      const int i = (j << 6) + lane;
+      const bool valid = i < count;
+      const uint32_t x = valid ? (uint32_t)v[i] : 0u;
+      const uint64_t Z = zm[j];
+      const uint64_t S = Z & ~((Z << 1) | (prevZ >> 63));
+      prevZ = Z;
+      uint32_t tok = 0;
+      int len = 0;
+      if (valid) {
+        if (x != 0) {
+          tok = 2 * x - 1;
+          len = varint_len(tok);
+        } else if ((S >> lane) & 1) {
+          const uint64_t mask = lane == 63 ? 0ull : (~0ull << (lane + 1));
+          uint64_t w = ~Z & mask;
+          int end;
+          if (w) {
+            end = (j << 6) + __ffsll((long long)w) - 1;
+          } else {
+            int jj = j + 1;
+            while (jj < 64 && zm[jj] == ~0ull) ++jj;
+            end = jj < 64 ? (jj << 6) + __ffsll((long long)~zm[jj]) - 1 : kShardSymbols;
+          }
+          end = min(end, count);
+          tok = 2u * (uint32_t)(end - i - 1);
+          len = varint_len(tok);
+        }
+      }
+      const int incl = wave_incl_scan(len);
+      if (len) write_varint_bytes(body + carry + incl - len, tok);
+      carry += shfl(incl, 63);
+    }
+  } else if (mode >= kModeRiceBase && mode <= kModeRiceBase + kMaxRice) {
+    // Rice (encode.cu:538-585): lane streams of symbols i%4, each code is q
+    // zero bits, a one, then k remainder bits LSB-first.
+    const int k = mode - kModeRiceBase;
+    const int lane_count = count < 4 ? count : 4;
+    uint32_t lane_off[4];
+    uint32_t acc = (uint32_t)(lane_count - 1) * 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      lane_off[r] = acc;
+      acc += rec.lane[r];
+    }
+    if (lane < lane_count - 1) {
+      const uint32_t len = rec.lane[lane];
+      body[lane * 4 + 0] = (uint8_t)len;
+      body[lane * 4 + 1] = (uint8_t)(len >> 8);
+      body[lane * 4 + 2] = (uint8_t)(len >> 16);
+      body[lane * 4 + 3] = (uint8_t)(len >> 24);
+    }
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t rmask = k ? ((1u << k) - 1) : 0u;
+    const int cls = lane & 3;
+    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    for (int j = 0; j < 64 && (j << 6) < count; ++j) {
+      const int i = (j << 6) + lane;
+      const bool valid = i < count;
+      const uint32_t x = valid ? (uint32_t)v[i] : 0u;
+      const uint32_t qv = x >> k;
+      const uint32_t len = valid ? qv + 1 + (uint32_t)k : 0u;
+      uint32_t incl = len;
+#pragma unroll
+      for (int d = 4; d < 64; d <<= 1) {
+        const uint32_t o = shfl_up(incl, d);
+        if (lane >= d) incl += o;
+      }
+      const uint32_t carry = cls == 0 ? c0 : cls == 1 ? c1 : cls == 2 ? c2 : c3;
+      if (valid) {
+        const uint32_t start = carry + incl - len;
+        const uint32_t pos = (3u + lane_off[cls]) * 8u + start + qv;
+        or_bits(buf, pos, 1u | ((x & rmask) << 1));
+      }
+      c0 += shfl(incl, 60);
+      c1 += shfl(incl, 61);
+      c2 += shfl(incl, 62);
+      c3 += shfl(incl, 63);
+    }
+  } else if (mode == kModeBlockPack) {
+    // block pack (encode.cu:614-682): per 128 symbols a width byte then
+    // ceil(n*w/8) bytes, values LSB-first at i*w.
+    int myw = 0, mycost = 0;
+    for (int b = 0; b < 32; ++b) {
+      const int i0 = (2 * b) << 6;
+      const uint32_t a = i0 + lane < count ? (uint32_t)v[i0 + lane] : 0u;
+      const uint32_t c = i0 + 64 + lane < count ? (uint32_t)v[i0 + 64 + lane] : 0u;
+      const uint32_t o = wave_or(a | c);
+      const int n = clampi(count - b * kBlockSymbols, 0, kBlockSymbols);
+      const int w = bit_length(o);
+      if (lane == b) {
+        myw = w;
+        mycost = n > 0 ? 1 + (n * w + 7) / 8 : 0;
+      }
+    }
+    const int incl = wave_incl_scan(mycost);
+    const int off = incl - mycost;
+    const int nblocks = (count + kBlockSymbols - 1) / kBlockSymbols;
+    if (lane < nblocks) body[off] = (uint8_t)myw;
+    __builtin_amdgcn_wave_barrier();
+    for (int j = 0; j < 64 && (j << 6) < count; ++j) {
+      const int i = (j << 6) + lane;
+      const int b = j >> 1;
+      const int w = shfl(myw, b);
+      const int boff = shfl(off, b);
+      if (i < count && w > 0) {
+        const uint32_t li = (uint32_t)(((j & 1) << 6) + lane);
+        const uint32_t pos = (3u + (uint32_t)boff + 1u) * 8u + li * (uint32_t)w;
+        or_bits(buf, pos, (uint32_t)v[i]);
+      }
+    }
+  } else {
+    // order-0 (encode.cu:587-611): sparse table, lane lengths 0..2, final
+    // states, lane bytes in decode order.
+    const uint8_t* sbase = scratch + (int64_t)s * kScratchBytes;
+    const int sp = rec.sparse_len;
+    for (int i = lane; i < sp; i += 64) body[i] = sbase[kScratchSparse + i];
+    if (lane < 3) {
+      const uint32_t len = rec.lane[lane];
+      for (int q = 0; q < 4; ++q) body[sp + lane * 4 + q] = (uint8_t)(len >> (8 * q));
+    }
+    if (lane < 4) {
+      const uint32_t stt = rec.states[lane];
+      for (int q = 0; q < 4; ++q) body[sp + 12 + lane * 4 + q] = (uint8_t)(stt >> (8 * q));
+    }
+    int off = sp + 28;
+    for (int r = 0; r < 4; ++r) {
+      const int len = rec.lane[r];
+      const uint8_t* srcl = sbase + kScratchLanes + (r + 1) * kLaneBytes - len;
+      for (int i = lane; i < len; i += 64) body[off + i] = srcl[i];
+      off += len;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) {
+    bytes[0] = (uint8_t)mode;
+    bytes[1] = (uint8_t)rec.body;
+    bytes[2] = (uint8_t)(rec.body >> 8);
+  }
+  __builtin_amdgcn_wave_barrier();
+  for (int i = lane; i < total; i += 64) out[i] = bytes[i];
+}
+
+// ------------------------------------------------------------ orchestration
+
+template <typename FT>
+static void encode_pass(const EncodeBuffers& b, hipStream_t stream) {
+  FT* folded = (FT*)b.folded;
+  if (b.any_lossy) {
+    {
+      FVH_TIMED("k_wavefront_enc", stream);
+      hipLaunchKernelGGL((k_wavefront<true, FT>), dim3((b.tile_total + 3) / 4), dim3(256), 0, stream,
+                         b.frames, b.states, b.tiles, b.tile_total, folded);
+    }
+  }
+  if (b.any_lossless) {
+    {
+      FVH_TIMED("k_lossless", stream);
+      hipLaunchKernelGGL((k_lossless<FT>), dim3(b.tile_total), dim3(256), 0, stream, b.frames,
+                         b.states, b.tiles, folded);
+    }
+  }
+  if (b.any_wide) {
+    {
+      FVH_TIMED("k_generic", stream);
+      hipLaunchKernelGGL((k_generic<true, FT>), dim3(b.tile_total), dim3(256), 0, stream, b.frames,
+                         b.states, b.tiles, folded, b.recon);
+    }
+  }
+  {
+    FVH_TIMED("k_analyze", stream);
+    hipLaunchKernelGGL((k_analyze<FT>), dim3((b.shard_total + 3) / 4), dim3(256), 0, stream,
+                       b.frames, b.states, b.tiles, b.shards, b.shard_total, folded, b.recs,
+                       b.scratch);
+  }
+  {
+    FVH_TIMED("k_rans", stream);
+    hipLaunchKernelGGL((k_rans<FT>), dim3((b.shard_total * 4 + 255) / 256), dim3(256), 0, stream,
+                       b.frames, b.states, b.tiles, b.shards, b.shard_total, folded, b.recs,
+                       b.scratch);
+  }
+}
+
+template <typename FT>
+static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
+  const dim3 fgrid((b.frame_count + 63) / 64);
+  const dim3 tgrid((b.tile_total + 3) / 4);
+  {
+    FVH_TIMED("k_state_init", stream);
+    hipLaunchKernelGGL(k_state_init, fgrid, dim3(64), 0, stream, b.frames, b.states, b.frame_count);
+  }
+  encode_pass<FT>(b, stream);
+  if (b.any_heuristic) {
+    {
+      FVH_TIMED("k_tile_scan", stream);
+      hipLaunchKernelGGL(k_tile_scan, tgrid, dim3(256), 0, stream, b.states, b.tiles, b.tile_total,
+                         b.recs, b.tile_len, 0);
+    }
+    {
+      FVH_TIMED("k_frame_finish", stream);
+      hipLaunchKernelGGL(k_frame_finish, dim3(b.frame_count), dim3(256), 0, stream, b.frames,
+                         b.states, b.tiles, b.tile_len, b.tile_off, b.results, 0);
+    }
+    {
+      FVH_TIMED("k_decide", stream);
+      hipLaunchKernelGGL(k_decide, fgrid, dim3(64), 0, stream, b.frames, b.states, b.frame_count, 1);
+    }
+    encode_pass<FT>(b, stream);
+    if (b.any_gray16) {
+      {
+        FVH_TIMED("k_tile_scan", stream);
+        hipLaunchKernelGGL(k_tile_scan, tgrid, dim3(256), 0, stream, b.states, b.tiles,
+                           b.tile_total, b.recs, b.tile_len, 0);
+      }
+      {
+        FVH_TIMED("k_frame_finish", stream);
+        hipLaunchKernelGGL(k_frame_finish, dim3(b.frame_count), dim3(256), 0, stream, b.frames,
+                           b.states, b.tiles, b.tile_len, b.tile_off, b.results, 0);
+      }
+      {
+        FVH_TIMED("k_decide", stream);
+        hipLaunchKernelGGL(k_decide, fgrid, dim3(64), 0, stream, b.frames, b.states, b.frame_count,
+                           2);
+      }
+      encode_pass<FT>(b, stream);
+    }
+    {
+      FVH_TIMED("k_decide", stream);
+      hipLaunchKernelGGL(k_decide, fgrid, dim3(64), 0, stream, b.frames, b.states, b.frame_count, 3);
+    }
+  }
+  {
+    FVH_TIMED("k_tile_scan", stream);
+    hipLaunchKernelGGL(k_tile_scan, tgrid, dim3(256), 0, stream, b.states, b.tiles, b.tile_total,
+                       b.recs, b.tile_len, 1);
+  }
+  {
+    FVH_TIMED("k_frame_finish", stream);
+    hipLaunchKernelGGL(k_frame_finish, dim3(b.frame_count), dim3(256), 0, stream, b.frames, b.states,
+                       b.tiles, b.tile_len, b.tile_off, b.results, 1);
+  }
+  {
+    FVH_TIMED("k_emit", stream);
+    hipLaunchKernelGGL((k_emit<FT>), dim3((b.shard_total + 3) / 4), dim3(256), 0, stream, b.frames,
+                       b.states, b.tiles, b.shards, b.shard_total, (const FT*)b.folded, b.recs,
+                       b.scratch, b.tile_off);
+  }
+}
+
+void launch_encode(const EncodeBuffers& b, hipStream_t stream) {
+  launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.tile_total, stream);
+  if (b.wide_folded) encode_all<uint32_t>(b, stream);
+  else encode_all<uint16_t>(b, stream);
+}
+
+// Decoder reconstruction shares the wavefront kernels.
+template <typename FT>
+void launch_reconstruct(const DecodeBuffers& b, hipStream_t stream) {
+  FT* folded = (FT*)b.folded;
+  {
+    FVH_TIMED("k_wavefront_dec", stream);
+    hipLaunchKernelGGL((k_wavefront<false, FT>), dim3((b.tile_total + 3) / 4), dim3(256), 0, stream,
+                       b.frames, b.states, b.tiles, b.tile_total, folded);
+  }
+  if (b.any_wide) {
+    {
+      FVH_TIMED("k_generic_dec", stream);
+      hipLaunchKernelGGL((k_generic<false, FT>), dim3(b.tile_total), dim3(256), 0, stream, b.frames,
+                         b.states, b.tiles, folded, (uint16_t*)nullptr);
+    }
+  }
+}
+
+template void launch_reconstruct<uint16_t>(const DecodeBuffers&, hipStream_t);
+template void launch_reconstruct<uint32_t>(const DecodeBuffers&, hipStream_t);
+
+}  // namespace fvh

@@ -1,0 +1,58 @@
+// fvh_kernels.h -- host-side launchers of the gfx950 kernels (internal).
+#pragma once
+
+#include "fvh_common.h"
+
+namespace fvh {
+
+constexpr int kRingMax = 4096;   // widest tile handled by the wave-per-tile wavefront
+
+struct EncodeBuffers {
+  FrameDesc* frames;
+  FrameState* states;
+  TileDesc* tiles;
+  ShardDesc* shards;
+  ShardRec* recs;
+  int64_t* tile_len;
+  int64_t* tile_off;
+  void* folded;          // uint16 or uint32 residuals, tile order
+  uint8_t* scratch;      // kScratchBytes per shard
+  uint16_t* recon;       // tile-order reconstruction for tiles wider than kRingMax (or null)
+  Results* results;
+  int frame_count;
+  int tile_total;
+  int shard_total;
+  int wide_folded;       // residuals stored as uint32 (16-bit frames present)
+  int any_heuristic;     // gray10 / gray16 lossy frames present
+  int any_gray16;
+  int any_lossless;
+  int any_lossy;
+  int any_wide;
+};
+
+struct DecodeBuffers {
+  FrameDesc* frames;
+  FrameState* states;
+  TileDesc* tiles;
+  ShardDesc* shards;
+  DShard* dshards;
+  void* folded;
+  Results* results;
+  int frame_count;
+  int tile_total;
+  int shard_total;
+  int wide_folded;
+  int any_wide;
+  int wavefront;
+};
+
+void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,
+                 int tile_total, hipStream_t stream);
+void launch_upload_frames(const FrameDesc* host_frames, int frame_count, FrameDesc* device_frames,
+                          hipStream_t stream);
+void launch_encode(const EncodeBuffers& b, hipStream_t stream);
+void launch_decode(const DecodeBuffers& b, hipStream_t stream);
+template <typename FT>
+void launch_reconstruct(const DecodeBuffers& b, hipStream_t stream);
+
+}  // namespace fvh

@@ -1,0 +1,73 @@
+// fvh_plan.hip -- batch geometry on the device.
+//
+// Tiles follow the canonical order of expected_tiles (cuda/csrc/encode.cu:41-69):
+// plane-major, then tile rows, then columns; yuv422 chroma planes use half
+// width and half nominal tile width; edge tiles are clipped.  Instead of the
+// reference's host-built metadata vectors and H2D copies
+// (cuda/csrc/encode.cu:739-767), each tile's origin, size, residual base and
+// first shard come from closed forms over the frame descriptor, one thread per
+// tile, so a batch of any number of frames is planned by one launch.
+#include "fvh_device.h"
+#include "fvh_kernels.h"
+#include "fvh_timing.h"
+
+namespace fvh {
+
+__global__ __launch_bounds__(256) void k_plan(const FrameDesc* __restrict__ frames,
+                                              int frame_count, TileDesc* __restrict__ tiles,
+                                              ShardDesc* __restrict__ shards, int tile_total) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tile_total) return;
+  int lo = 0, hi = frame_count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (frames[mid].tile_begin <= t) lo = mid; else hi = mid - 1;
+  }
+  const FrameDesc& f = frames[lo];
+  const int local = t - f.tile_begin;
+  const int plane = local >= f.plane_tile_begin[2] ? 2 : (local >= f.plane_tile_begin[1] ? 1 : 0);
+  const int i = local - f.plane_tile_begin[plane];
+  const int r = i / f.cols[plane];
+  const int c = i - r * f.cols[plane];
+  const int64_t ntw = f.ntw[plane];
+  const int64_t pw = f.plane_width[plane];
+  const int64_t x = (int64_t)c * ntw;
+  const int64_t y = (int64_t)r * f.tile_h;
+  const int64_t w = min(ntw, pw - x);
+  const int64_t h = min((int64_t)f.tile_h, f.height - y);
+  TileDesc td;
+  td.folded_base = f.folded_base + f.plane_sample_base[plane] + y * pw + x * h;
+  td.src_offset = y * f.pitch[plane] + x;
+  td.dst_offset = y * pw + x;
+  td.frame = lo;
+  td.plane = plane;
+  td.x = (int32_t)x;
+  td.y = (int32_t)y;
+  td.width = (int32_t)w;
+  td.height = (int32_t)h;
+  const int64_t full_tile_shards = (ntw * h + kShardSymbols - 1) / kShardSymbols;
+  const int64_t first = f.shard_begin + f.plane_shard_base[plane] + (int64_t)r * f.row_shards[plane] +
+      (int64_t)c * full_tile_shards;
+  const int64_t count = (w * h + kShardSymbols - 1) / kShardSymbols;
+  td.first_shard = (int32_t)first;
+  td.shard_count = (int32_t)count;
+  tiles[t] = td;
+  for (int64_t s = 0; s < count; ++s) {
+    ShardDesc sd;
+    sd.tile = t;
+    sd.local = (int32_t)s;
+    shards[first + s] = sd;
+  }
+}
+
+void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,
+                 int tile_total, hipStream_t stream) {
+  if (tile_total <= 0) return;
+  {
+    FVH_TIMED("k_plan", stream);
+    hipLaunchKernelGGL(k_plan, dim3((tile_total + 255) / 256), dim3(256), 0, stream, frames,
+                       frame_count, tiles, shards, tile_total);
+  }
+}
+
+}  // namespace fvh

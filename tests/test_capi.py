@@ -1,0 +1,166 @@
+"""CPU tests of the C-ABI library: exports, host-side validation, and the exact
+integer division identities the kernels rely on.  No GPU call is made."""
+
+import ctypes
+import re
+import struct
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from helpers import noisy_planes, source_planes
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+@pytest.fixture(scope="module")
+def native():
+    from fastvid_amd import _native
+    _native.lib()
+    return _native
+
+
+def test_library_exports_every_declared_symbol(native):
+    header = (ROOT / "include" / "fastvid_hip.h").read_text()
+    names = re.findall(r"^\s*(?:int|const char\*)\s+(fvh_\w+)\s*\(", header, re.M)
+    assert len(names) >= 9
+    handle = native.lib()
+    for name in names:
+        assert hasattr(handle, name), name
+
+
+def _frame(native, planes_shape, layout, bit_depth=8, quality=90, tile=(256, 128), fps=(24, 1)):
+    f = native.EncodeFrame()
+    f.plane_count = len(planes_shape)
+    for i, (h, w) in enumerate(planes_shape):
+        f.planes[i] = 0x1000 * (i + 1)  # never dereferenced by the planner
+        f.pitches[i] = w
+        f.plane_widths[i] = w
+        f.plane_heights[i] = h
+    f.layout, f.bit_depth, f.quality = layout, bit_depth, quality
+    f.fps_numerator, f.fps_denominator = fps
+    f.tile_width, f.tile_height = tile
+    return f
+
+
+def _plan(native, frame):
+    msg = native.message_buffer()
+    ws, out = ctypes.c_uint64(), ctypes.c_uint64()
+    offsets = (ctypes.c_uint64 * 1)()
+    rc = native.lib().fvh_encode_plan(ctypes.addressof(frame), 1, ctypes.byref(ws),
+                                      ctypes.byref(out), offsets, msg, len(msg))
+    return rc, msg.value.decode(), ws.value, out.value
+
+
+@pytest.mark.parametrize("kwargs,message", [
+    ({"layout": 3}, "layout must be gray, yuv422, or rgb444"),
+    ({"bit_depth": 9}, "bit_depth must be 8, 10, 12, or 16"),
+    ({"quality": 101}, "quality must be in 1..=100"),
+    ({"fps": (0, 1)}, "frame-rate components must be nonzero u32 values"),
+    ({"tile": (70000, 8)}, "tile dimensions must be nonzero u16 values"),
+    ({"tile": (1, 1)}, "too many tiles"),
+])
+def test_encode_plan_rejects_like_reference(native, kwargs, message):
+    shape = [(2160, 3840)]
+    frame = _frame(native, shape, kwargs.pop("layout", 0), **kwargs)
+    rc, msg, _, _ = _plan(native, frame)
+    assert rc == native.FVH_EARG and msg == message
+
+
+def test_encode_plan_plane_checks(native):
+    rc, msg, _, _ = _plan(native, _frame(native, [(10, 10)], 2))
+    assert msg == "plane count does not match layout"
+    rc, msg, _, _ = _plan(native, _frame(native, [(10, 11), (10, 5), (10, 5)], 1))
+    assert msg == "plane dimensions do not match declared layout"
+    rc, msg, ws, out = _plan(native, _frame(native, [(10, 11), (10, 6), (10, 6)], 1))
+    assert rc == 0 and ws > 0 and out > 0
+
+
+def test_output_bound_covers_oracle_streams(native, oracle):
+    """The planner's stream capacity is never below a real stream's length."""
+    for layout, bd, q in (("rgb444", 8, 90), ("gray", 16, 100), ("yuv422", 10, 95)):
+        source = noisy_planes(layout, 120, 90, bd, seed=3, sigma=(1 << bd) / 4)
+        stream = oracle.encode(source, layout=layout, bit_depth=bd, quality=q, tile_size=(32, 32))
+        shapes = [p.shape for p in source]
+        frame = _frame(native, shapes, {"gray": 0, "yuv422": 1, "rgb444": 2}[layout], bd, q, (32, 32))
+        rc, msg, ws, out = _plan(native, frame)
+        assert rc == 0 and out >= len(stream)
+
+
+def _header(native, stream):
+    info = native.StreamInfo()
+    msg = native.message_buffer()
+    buf = ctypes.create_string_buffer(bytes(stream[:32]).ljust(32, b"\0"), 32)
+    rc = native.lib().fvh_decode_header(buf, len(stream), ctypes.byref(info), msg, len(msg))
+    return rc, msg.value.decode(), info
+
+
+def test_decode_header_matches_oracle(native, oracle):
+    for layout, bd, q in (("gray", 10, 90), ("yuv422", 8, 100), ("rgb444", 16, 50), ("gray", 16, 95)):
+        source = noisy_planes(layout, 77, 41, bd, seed=4)
+        stream = oracle.encode(source, layout=layout, bit_depth=bd, quality=q, tile_size=(32, 16))
+        rc, msg, info = _header(native, stream)
+        ref = oracle.parse_header(stream)
+        assert rc == 0
+        for key in ("layout", "quality", "bit_depth", "width", "height", "tile_count",
+                    "total_samples", "step", "max_sample", "use_med"):
+            assert getattr(info, key) == ref[key], key
+
+
+def test_decode_header_and_walk_messages_match_oracle(native, oracle):
+    source = source_planes("yuv422", 263, 133, 10)
+    stream = oracle.encode(source, layout="yuv422", bit_depth=10, quality=100)
+    first = struct.unpack_from("<Q", stream, 52)[0]
+    rng = np.random.default_rng(0)
+    mutants = [stream[:20], stream[:-1], stream + b"\0"]
+    for index in [0, 4, 5, 6, 7, 8, 16, 28, 32, 33, 34, 35, 36, 52, 60, first, first + 1, first + 2]:
+        for value in (0, 1, 7, 0x40, 0x80, 255):
+            m = bytearray(stream)
+            m[index] = value
+            mutants.append(bytes(m))
+    for _ in range(200):
+        m = bytearray(stream)
+        m[int(rng.integers(0, 32 + 32 * 8))] = int(rng.integers(0, 256))
+        mutants.append(bytes(m))
+    for m in mutants:
+        try:
+            oracle.decode(m)
+            expected = None
+        except oracle.OracleError as err:
+            expected = str(err)
+        rc, msg, info = _header(native, m)
+        if rc == 0:
+            buf = ctypes.create_string_buffer(m, len(m))
+            cmsg = native.message_buffer()
+            rc = native.lib().fvh_decode_check_host(buf, len(m), ctypes.byref(info), cmsg, len(cmsg))
+            msg = cmsg.value.decode()
+        if expected is None or expected.startswith("malformed entropy payload"):
+            assert rc == 0, (msg, expected)
+        else:
+            assert rc != 0 and msg == expected
+
+
+def test_quantiser_reciprocal_is_exact():
+    """floor(n/d) == mul_hi(n, ceil(2^32/d)) for every step d the format can
+    produce (2..4225) and every numerator |r| + d/2 < 2^17."""
+    n = np.arange(1 << 17, dtype=np.uint64)
+    for d in range(2, 4226):
+        m = np.uint64(((1 << 32) + d - 1) // d)
+        assert np.array_equal((n * m) >> np.uint64(32), n // np.uint64(d)), d
+
+
+def test_rans_reciprocal_is_exact():
+    """floor(x/f) == mul_hi(x, ceil(2^(31+c)/f)) >> (c-1), c = ceil(log2 f), for
+    x < 2^31 (the encoder state after renormalisation) and 2 <= f <= 4096."""
+    rng = np.random.default_rng(7)
+    x_rand = rng.integers(0, 1 << 31, size=20000, dtype=np.uint64)
+    for f in range(2, 4097):
+        c = int(f - 1).bit_length()
+        m = ((1 << (31 + c)) + f - 1) // f
+        assert m < (1 << 32)
+        k = np.arange(1, (1 << 31) // f, max(1, ((1 << 31) // f) // 64), dtype=np.uint64)
+        x = np.concatenate([x_rand, k * np.uint64(f), k * np.uint64(f) - np.uint64(1),
+                            np.array([(1 << 31) - 1], np.uint64)])
+        q = ((x * np.uint64(m)) >> np.uint64(32)) >> np.uint64(c - 1)
+        assert np.array_equal(q, x // np.uint64(f)), f

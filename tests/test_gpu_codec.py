@@ -1,0 +1,160 @@
+"""GPU parity tests of the HIP codec against the CPU oracle.
+
+The first three tests restate the reference suite (cuda/tests/test_codec.py:52-115)
+on the drop-in API; the rest compare every stream byte-for-byte with the
+oracle's and every decode with the oracle's reconstruction.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import noisy_planes, shard_modes, source_planes
+
+pytestmark = pytest.mark.gpu
+
+
+def _cuda(planes):
+    return tuple(torch.from_numpy(p.astype(np.int32)).to(torch.uint16).cuda() for p in planes)
+
+
+def _host(t):
+    return t.cpu().numpy()
+
+
+def _bytes(stream):
+    return bytes(stream.cpu().numpy().tobytes())
+
+
+@pytest.fixture(scope="module")
+def fc():
+    import fastvid_cuda
+    return fastvid_cuda
+
+
+@pytest.mark.parametrize("layout", ("gray", "yuv422", "rgb444"))
+@pytest.mark.parametrize("bit_depth", (8, 10, 12, 16))
+def test_q100_roundtrip_is_exact_and_emits_version_one(fc, oracle, layout, bit_depth):
+    source = source_planes(layout, 67, 35, bit_depth)
+    cuda_source = _cuda(source)
+    first = fc.encode(cuda_source, layout=layout, bit_depth=bit_depth, quality=100)
+    second = fc.encode(cuda_source, layout=layout, bit_depth=bit_depth, quality=100)
+    assert first.is_cuda and first.dtype == torch.uint8
+    assert _bytes(first) == _bytes(second)
+    assert int(first[4].item()) == 1
+    assert _bytes(first) == oracle.encode(source, layout=layout, bit_depth=bit_depth, quality=100)
+    for placement in (first.cpu(), first):
+        for predictor in ("wavefront", "serial"):
+            actual = fc.decode(placement, predictor=predictor)
+            assert len(actual) == len(source)
+            for result, expected in zip(actual, source):
+                assert result.is_cuda and result.dtype == torch.uint16
+                assert np.array_equal(_host(result), expected)
+
+
+def test_lossy_stream_is_deterministic_and_exercises_entropy_modes(fc, oracle):
+    selected = set()
+    for quality, pattern in ((90, "ramp"), (100, "zero"), (100, "polynomial")):
+        source = source_planes("yuv422", 263, 133, 10, pattern)
+        stream = fc.encode(_cuda(source), layout="yuv422", bit_depth=10, quality=quality)
+        payload = _bytes(stream)
+        assert payload == oracle.encode(source, layout="yuv422", bit_depth=10, quality=quality)
+        selected.update(shard_modes(payload))
+        wavefront = fc.decode(stream, predictor="wavefront")
+        serial = fc.decode(stream, predictor="serial")
+        assert all(torch.equal(a.cpu(), b.cpu()) for a, b in zip(wavefront, serial))
+        expected = oracle.decode(payload)
+        assert all(np.array_equal(_host(a), b) for a, b in zip(wavefront, expected))
+    assert 0 in selected
+    assert 18 in selected
+    assert any(1 <= mode <= 17 for mode in selected)
+
+
+def test_decode_rejects_other_versions_and_malformed_streams(fc):
+    import struct
+    source = source_planes("yuv422", 263, 133, 10)
+    stream = bytearray(fc.encode(_cuda(source), layout="yuv422", bit_depth=10,
+                                 quality=100).cpu().numpy().tobytes())
+    first_payload = struct.unpack_from("<Q", stream, 32 + 20)[0]
+    mutations = []
+    old_version = bytearray(stream)
+    old_version[4] = 7
+    mutations.append(old_version)
+    bad_directory = bytearray(stream)
+    bad_directory[32 + 3] = 1
+    mutations.append(bad_directory)
+    bad_offset = bytearray(stream)
+    struct.pack_into("<Q", bad_offset, 32 + 20, first_payload + 1)
+    mutations.append(bad_offset)
+    bad_mode = bytearray(stream)
+    bad_mode[first_payload] = 255
+    mutations.append(bad_mode)
+    mutations.append(bytearray(stream[:-1]))
+    mutations.append(bytearray(stream + b"\x00"))
+    for malformed in mutations:
+        tensor = torch.frombuffer(malformed, dtype=torch.uint8)
+        for placement in (tensor.cuda(), tensor):
+            with pytest.raises(RuntimeError):
+                fc.decode(placement)
+
+
+CASES = [
+    # layout, width, height, bit_depth, quality, tile
+    ("gray", 200, 150, 8, 90, (256, 128)),
+    ("gray", 300, 170, 10, 90, (64, 64)),       # gray10 lossy: MED heuristic
+    ("gray", 300, 170, 10, 60, (128, 32)),
+    ("gray", 210, 190, 12, 70, (48, 80)),
+    ("gray", 257, 131, 16, 87, (256, 128)),     # gray16: refinement heuristic
+    ("gray", 257, 131, 16, 95, (100, 60)),
+    ("yuv422", 333, 111, 8, 90, (256, 128)),
+    ("yuv422", 333, 111, 10, 75, (64, 32)),
+    ("yuv422", 180, 99, 16, 50, (96, 96)),
+    ("rgb444", 190, 140, 8, 90, (256, 128)),    # forced order-0
+    ("rgb444", 190, 140, 10, 85, (40, 40)),
+    ("rgb444", 130, 97, 12, 100, (32, 32)),
+    ("rgb444", 96, 64, 16, 40, (256, 128)),
+    ("gray", 1, 1, 8, 100, (256, 128)),
+    ("gray", 5, 3, 10, 20, (2, 2)),
+    ("rgb444", 7, 300, 8, 90, (3, 64)),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[1]}x{c[2]}-{c[3]}b-q{c[4]}-t{c[5][0]}x{c[5][1]}" for c in CASES])
+def test_stream_parity_with_oracle(fc, oracle, case):
+    layout, width, height, bit_depth, quality, tile = case
+    source = noisy_planes(layout, width, height, bit_depth, seed=width * 7 + height)
+    stream = fc.encode(_cuda(source), layout=layout, bit_depth=bit_depth, quality=quality,
+                       tile_size=tile)
+    expected = oracle.encode(source, layout=layout, bit_depth=bit_depth, quality=quality,
+                             tile_size=tile)
+    got = _bytes(stream)
+    assert len(got) == len(expected)
+    assert got == expected
+    decoded = fc.decode(stream)
+    reference = oracle.decode(expected)
+    for a, b in zip(decoded, reference):
+        assert np.array_equal(_host(a), b)
+
+
+def test_out_of_range_sample_fails_like_reference(fc):
+    plane = np.full((20, 30), 300, np.uint16)
+    with pytest.raises(RuntimeError, match="status 1"):
+        fc.encode(_cuda((plane,)), layout="gray", bit_depth=8, quality=100)
+    with pytest.raises(RuntimeError, match="status 1"):
+        fc.encode(_cuda((plane,)), layout="gray", bit_depth=8, quality=50)
+
+
+def test_argument_errors(fc):
+    plane = torch.zeros((8, 8), dtype=torch.uint16, device="cuda")
+    with pytest.raises(RuntimeError, match="bit_depth"):
+        fc.encode(plane, bit_depth=9)
+    with pytest.raises(RuntimeError, match="quality"):
+        fc.encode(plane, bit_depth=8, quality=0)
+    with pytest.raises(RuntimeError, match="tile dimensions"):
+        fc.encode(plane, bit_depth=8, tile_size=(0, 5))
+    with pytest.raises(RuntimeError, match="CUDA tensors"):
+        fc.encode(plane.cpu(), bit_depth=8)
+    with pytest.raises(KeyError):
+        fc.encode(plane, layout="bgr")
+    with pytest.raises(ValueError):
+        fc.decode(torch.zeros(40, dtype=torch.uint8), predictor="diagonal")
