@@ -49,17 +49,50 @@ def make_frames(count):
         return list(pool.map(lambda s: synthetic.rgb_frame(s, WIDTH, HEIGHT), range(1, count + 1)))
 
 
-def algorithmic_bytes(samples, stream_bytes):
-    """Per-kernel algorithmic bytes: residuals at their minimal width (u16
-    for bit depth <= 12), uint16 samples, stream bytes."""
+def mode_split(streams):
+    """Samples and body bytes of order-0 shards vs the other modes, read from
+    the streams' headers (tile count at byte 28), directories and record
+    headers."""
+    o0_samples = o0_bytes = other_samples = other_bytes = 0
+    for st in streams:
+        b = st.cpu().numpy().tobytes()
+        tiles = int.from_bytes(b[28:32], "little")
+        for t in range(tiles):
+            e = 32 + 32 * t
+            w = int.from_bytes(b[e + 12:e + 16], "little")
+            h = int.from_bytes(b[e + 16:e + 20], "little")
+            cur = int.from_bytes(b[e + 20:e + 28], "little")
+            left = w * h
+            while left > 0:
+                n = min(left, 4096)
+                mode = b[cur]
+                length = b[cur + 1] | (b[cur + 2] << 8)
+                if mode == 19:
+                    o0_samples += n
+                    o0_bytes += length + 3
+                else:
+                    other_samples += n
+                    other_bytes += length + 3
+                cur += 3 + length
+                left -= n
+    return o0_samples, o0_bytes, other_samples, other_bytes
+
+
+def algorithmic_bytes(samples, stream_bytes, split):
+    """Per-kernel algorithmic bytes per step: residuals at their minimal width
+    (u16 for bit depth <= 12), uint16 samples, stream bytes."""
     s, b = samples, stream_bytes
+    o0_s, o0_b, ot_s, ot_b = split
     return {
-        "k_wavefront_enc": 2 * s + 2 * s,   # samples in, residuals out
+        "k_band_enc": 2 * s + 2 * s,        # samples in, residuals out
+        "k_wavefront_enc": 2 * s + 2 * s,
         "k_analyze": 2 * s,                 # residuals in
         "k_rans": 2 * s + b,                # residuals in, lane bytes out
         "k_emit": 2 * s + 2 * b,            # residuals + lane bytes in, stream out
-        "k_entropy": b + 2 * s,             # stream in, residuals out
-        "k_wavefront_dec": 2 * s + 2 * s,   # residuals in, samples out
+        "k_dec_o0": o0_b + 2 * o0_s,        # order-0 bodies in, residuals out
+        "k_dec_other": ot_b + 2 * ot_s,     # other bodies in, residuals out
+        "k_band_dec": 2 * s + 2 * s,        # residuals in, samples out
+        "k_wavefront_dec": 2 * s + 2 * s,
     }
 
 
@@ -172,7 +205,8 @@ def main():
 
     # ---- roofline of the dominant kernel (HIP events on the launch stream)
     samples = 3 * WIDTH * HEIGHT * args.frames
-    alg = algorithmic_bytes(samples, stream_bytes)
+    split = mode_split(streams)
+    alg = algorithmic_bytes(samples, stream_bytes, split)
     stages = {}
     for name, (launches, total_ms) in kernels.items():
         per = total_ms / max(launches, 1)
@@ -189,6 +223,11 @@ def main():
         d = stages[dominant]
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": d["gbs"], "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": d["hbm_frac"], "traffic": None}
+    predictor = None
+    if "k_band_enc" in stages and "gbs" in stages["k_band_enc"]:
+        d = stages["k_band_enc"]
+        predictor = {"kernel": "k_band_enc", "achieved": d["gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": d["hbm_frac"]}
 
     # ---- CPU baseline: the oracle on host cores, bounded sample
     cpu = None
@@ -235,6 +274,8 @@ def main():
         "stream_bytes_per_frame": stream_bytes // args.frames,
         "verify": verify,
         "roofline": roofline,
+        "predictor_roofline": predictor,
+        "order0_share": round(split[0] / max(1, split[0] + split[2]), 4),
         "stages": stages,
         "cpu_baseline": cpu,
         "setup_s": round(gen_s, 1),

@@ -131,6 +131,18 @@ int64_t tile_count_closed(int64_t width, int64_t height, int64_t tw, int64_t th,
   return rows * (luma + 2 * ceil_div((width + 1) / 2, (tw + 1) / 2));
 }
 
+// Frames whose every tile fits the band wavefront's vector accesses: planes
+// 16-byte aligned, pitches, plane widths and nominal tile widths multiples of
+// 8 samples, tiles no wider than kBandRingMax.
+int band_eligible(const FrameDesc& f, const void* const* planes) {
+  for (int p = 0; p < f.plane_count; ++p) {
+    if (((uintptr_t)planes[p] & 15) != 0) return 0;
+    if (f.pitch[p] % 8 != 0 || f.plane_width[p] % 8 != 0 || f.ntw[p] % 8 != 0) return 0;
+    if (std::min<int64_t>(f.ntw[p], f.plane_width[p]) > kBandRingMax) return 0;
+  }
+  return 1;
+}
+
 struct EncodePlan {
   std::vector<FrameDesc> frames;
   std::vector<int64_t> output_offsets;
@@ -138,6 +150,7 @@ struct EncodePlan {
   int64_t tiles = 0, shards = 0, samples = 0;
   bool wide_folded = false, any_heuristic = false, any_gray16 = false;
   bool any_lossless = false, any_lossy = false, any_wide = false;
+  bool any_band = false, any_nonband = false, any_med = false, any_grad = false;
   WorkspaceLayout ws = {};
 };
 
@@ -209,7 +222,9 @@ int build_encode_plan(const fvh_encode_frame* in, int32_t count, EncodePlan& pla
       f.pitch[p] = p < fr.plane_count ? fr.pitches[p] : 0;
       f.out_plane[p] = nullptr;
     }
+    plan.samples = (plan.samples + 7) & ~int64_t{7};  // 16-byte aligned residual rows
     f.folded_base = plan.samples;
+    f.band = band_eligible(f, fr.planes);
     f.tile_begin = (int32_t)plan.tiles;
     f.tile_count = (int32_t)g.tiles;
     f.shard_begin = (int32_t)plan.shards;
@@ -237,6 +252,12 @@ int build_encode_plan(const fvh_encode_frame* in, int32_t count, EncodePlan& pla
     plan.any_lossless |= f.step0 == 1;
     plan.any_lossy |= f.step0 > 1;
     plan.any_wide |= g.wide;
+    if (f.step0 > 1) {
+      plan.any_band |= f.band != 0;
+      plan.any_nonband |= f.band == 0;
+    }
+    plan.any_med |= fr.layout != 0 || f.heuristic == 1;
+    plan.any_grad |= fr.layout == 0;
   }
   if (plan.tiles > INT32_MAX / 2 || plan.shards > INT32_MAX / 4) {
     set_msg(msg, msg_len, "batch is too large");
@@ -257,7 +278,7 @@ int build_encode_plan(const fvh_encode_frame* in, int32_t count, EncodePlan& pla
   w.recs = take(plan.shards * (int64_t)sizeof(ShardRec));
   w.tile_len = take(plan.tiles * 8);
   w.tile_off = take(plan.tiles * 8);
-  w.folded = take(plan.samples * (plan.wide_folded ? 4 : 2));
+  w.folded = take((plan.samples + 8) * (plan.wide_folded ? 4 : 2));
   w.scratch = take(plan.shards * (int64_t)kScratchBytes);
   w.recon = plan.any_wide ? take(plan.samples * 2) : -1;
   w.dshards = -1;
@@ -321,6 +342,10 @@ int fvh_encode_launch(const fvh_encode_frame* frames, int32_t frame_count, void*
   b.any_lossless = plan.any_lossless;
   b.any_lossy = plan.any_lossy;
   b.any_wide = plan.any_wide;
+  b.any_band = plan.any_band;
+  b.any_nonband = plan.any_nonband;
+  b.any_med = plan.any_med;
+  b.any_grad = plan.any_grad;
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
   launch_upload_frames(plan.frames.data(), frame_count, b.frames, s);
@@ -457,7 +482,7 @@ int fvh_decode_check_host(const uint8_t* bytes, int64_t size, const fvh_stream_i
 }
 
 struct DecodePlanLayout {
-  int64_t frames, states, tiles, shards, dshards, folded, total;
+  int64_t frames, states, tiles, shards, dshards, lists, folded, total;
   int64_t tiles_n, shards_n, samples;
   bool wide_folded, any_wide;
 };
@@ -467,7 +492,7 @@ static DecodePlanLayout decode_layout(const fvh_stream_info* infos, int32_t coun
   for (int32_t i = 0; i < count; ++i) {
     L.tiles_n += infos[i].tile_count;
     L.shards_n += infos[i].total_shards;
-    L.samples += infos[i].total_samples;
+    L.samples = ((L.samples + 7) & ~int64_t{7}) + infos[i].total_samples;
     L.wide_folded |= infos[i].bit_depth == 16;
     const int64_t tw = infos[i].tile_width;
     const int64_t ctw = infos[i].layout == 1 ? (tw + 1) / 2 : tw;
@@ -485,7 +510,8 @@ static DecodePlanLayout decode_layout(const fvh_stream_info* infos, int32_t coun
   L.tiles = take(L.tiles_n * (int64_t)sizeof(TileDesc));
   L.shards = take(L.shards_n * (int64_t)sizeof(ShardDesc));
   L.dshards = take(L.shards_n * (int64_t)sizeof(DShard));
-  L.folded = take(L.samples * (L.wide_folded ? 4 : 2));
+  L.lists = take((2 * L.shards_n + 4) * 4);
+  L.folded = take((L.samples + 8) * (L.wide_folded ? 4 : 2));
   L.total = off;
   return L;
 }
@@ -520,6 +546,7 @@ int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
   }
   std::vector<FrameDesc> frames(count);
   int64_t tiles = 0, shards = 0, samples = 0;
+  bool any_band = false, any_nonband = false, any_med = false, any_grad = false;
   for (int32_t i = 0; i < count; ++i) {
     const fvh_stream_info& h = infos[i];
     FrameDesc& f = frames[i];
@@ -534,7 +561,14 @@ int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
       f.plane[p] = nullptr;
       f.pitch[p] = f.plane_width[p];
     }
+    samples = (samples + 7) & ~int64_t{7};
     f.folded_base = samples;
+    const void* outs[3] = {f.out_plane[0], f.out_plane[1], f.out_plane[2]};
+    f.band = band_eligible(f, outs);
+    any_band |= f.band != 0;
+    any_nonband |= f.band == 0;
+    any_med |= h.use_med != 0;
+    any_grad |= h.use_med == 0;
     f.tile_begin = (int32_t)tiles;
     f.tile_count = (int32_t)g.tiles;
     f.shard_begin = (int32_t)shards;
@@ -556,6 +590,8 @@ int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
   b.tiles = (TileDesc*)(ws + L.tiles);
   b.shards = (ShardDesc*)(ws + L.shards);
   b.dshards = (DShard*)(ws + L.dshards);
+  b.lists = (int32_t*)(ws + L.lists);
+  b.list_count = b.lists + 2 * L.shards_n;
   b.folded = ws + L.folded;
   b.results = (Results*)results;
   b.frame_count = count;
@@ -564,6 +600,10 @@ int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
   b.wide_folded = L.wide_folded;
   b.any_wide = L.any_wide;
   b.wavefront = wavefront;
+  b.any_band = any_band;
+  b.any_nonband = any_nonband;
+  b.any_med = any_med;
+  b.any_grad = any_grad;
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
   launch_upload_frames(frames.data(), count, b.frames, s);

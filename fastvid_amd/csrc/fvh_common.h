@@ -32,11 +32,12 @@ constexpr uint32_t kRansL = 1u << 23;
 constexpr int kMaxRecordBytes = 8739;
 
 // Per-shard order-0 scratch layout (bytes).
-constexpr int kScratchTable = 0;        // 512 x u64 encode entries
-constexpr int kScratchSparse = 4096;    // sparse table bytes (<= 1024)
-constexpr int kScratchLanes = 5120;     // 4 lanes x kLaneBytes, filled from the end
+constexpr int kScratchTable = 0;        // 256 x {freq|cum<<13|c<<25, reciprocal} by rank
+constexpr int kScratchSparse = 2048;    // sparse table bytes (<= 1024)
+constexpr int kScratchRanks = 3072;     // rank of every symbol, one byte each (4096)
+constexpr int kScratchLanes = 7168;     // 4 lanes x kLaneBytes, filled from the end
 constexpr int kLaneBytes = 1536;        // proven bound for 1024 symbols
-constexpr int kScratchBytes = kScratchLanes + 4 * kLaneBytes;  // 11264
+constexpr int kScratchBytes = kScratchLanes + 4 * kLaneBytes;  // 13312
 
 // Frame-level status codes (first error wins).
 constexpr int kStatusSampleRange = 1;     // encoder: sample > max (encode.cu:160-162)
@@ -77,7 +78,7 @@ struct FrameDesc {
   int32_t use_med;             // decode: predictor choice
   int32_t step;                // decode: step from the header
   int32_t header_flags;        // decode: header byte 7 flag bits
-  int32_t pad;
+  int32_t band;                // tiles eligible for the band wavefront (k_band)
 };
 
 // Per-frame state written on the device during a call.

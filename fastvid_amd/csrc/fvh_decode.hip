@@ -23,8 +23,13 @@ __device__ __forceinline__ uint64_t rd_u64(const uint8_t* p) {
   return (uint64_t)rd_u32(p) | ((uint64_t)rd_u32(p + 4) << 32);
 }
 
-__global__ void k_dstate_init(FrameState* __restrict__ states, int frame_count) {
+__global__ void k_dstate_init(FrameState* __restrict__ states, int frame_count,
+                              int32_t* __restrict__ list_count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    list_count[0] = 0;
+    list_count[1] = 0;
+  }
   if (i >= frame_count) return;
   FrameState s = {};
   s.active = 1;
@@ -35,13 +40,16 @@ __global__ void k_dstate_init(FrameState* __restrict__ states, int frame_count) 
 __global__ __launch_bounds__(256) void k_parse(const FrameDesc* __restrict__ frames,
                                                FrameState* __restrict__ states,
                                                const TileDesc* __restrict__ tiles, int tile_total,
-                                               DShard* __restrict__ dshards) {
+                                               DShard* __restrict__ dshards,
+                                               int32_t* __restrict__ lists,
+                                               int32_t* __restrict__ list_count, int shard_total,
+                                               char* gbase) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= tile_total) return;
   const TileDesc td = tiles[t];
   const FrameDesc& f = frames[td.frame];
   FrameState* st = &states[td.frame];
-  const uint8_t* enc = f.stream;
+  const uint8_t* enc = as_global(gbase, (const uint8_t*)f.stream);
   const int64_t size = f.stream_capacity;
   const int local = t - f.tile_begin;
   const int64_t T = f.tile_count;
@@ -86,6 +94,7 @@ __global__ __launch_bounds__(256) void k_parse(const FrameDesc* __restrict__ fra
   const int64_t samples = (int64_t)td.width * td.height;
   int64_t cursor = payload_offset;
   int64_t decoded = 0;
+  int n_order0 = 0;
   const int64_t tile_end = payload_offset + payload_length;
   for (int ls = 0; ls < td.shard_count; ++ls) {
     if (cursor > tile_end - 3) {
@@ -108,10 +117,22 @@ __global__ __launch_bounds__(256) void k_parse(const FrameDesc* __restrict__ fra
     d.body_length = (int32_t)body_length;
     d.count = (int32_t)n;
     dshards[td.first_shard + ls] = d;
+    n_order0 += mode == kModeOrder0;
     decoded += n;
     cursor = body_offset + body_length;
   }
-  if (decoded != samples || cursor != tile_end) set_status(st, kStatusShardHeader);
+  if (decoded != samples || cursor != tile_end) {
+    set_status(st, kStatusShardHeader);
+    return;
+  }
+  // file the tile's shards by decoder kernel
+  int at0 = n_order0 ? atomicAdd(&list_count[0], n_order0) : 0;
+  int at1 = td.shard_count - n_order0 ? atomicAdd(&list_count[1], td.shard_count - n_order0) : 0;
+  for (int ls = 0; ls < td.shard_count; ++ls) {
+    const int shard = td.first_shard + ls;
+    if (dshards[shard].mode == kModeOrder0) lists[at0++] = shard;
+    else lists[shard_total + at1++] = shard;
+  }
 }
 
 // Canonical LEB128 with at most five bytes (decode.cu:244-272, 407-426).
@@ -188,199 +209,273 @@ struct BitReader {
   }
 };
 
-// One wave64 per shard.  Modes: zero-run (one lane), Rice (one lane per Rice
-// lane), block-pack (one lane per 128-symbol block), order-0 rANS (table
-// parse by one lane, LUT striped over the wave, one lane per rANS lane).
+// One zero-run, Rice or block-pack body (decode.cu:222-399) decoded by one
+// wave; `body` is either the LDS copy or, for bodies too long to stage, the
+// stream itself (the function is instantiated for each so LDS reads stay LDS
+// reads).
 template <typename FT>
-__global__ __launch_bounds__(256) void k_entropy(const FrameDesc* __restrict__ frames,
-                                                 FrameState* __restrict__ states,
-                                                 const TileDesc* __restrict__ tiles,
-                                                 const ShardDesc* __restrict__ shards,
-                                                 const DShard* __restrict__ dshards,
-                                                 int shard_total, FT* __restrict__ folded) {
-  __shared__ uint16_t values_all[4][256];
-  __shared__ uint16_t freq_all[4][256];
-  __shared__ uint16_t cum_all[4][256];
-  __shared__ uint8_t lut_all[4][4096];
-  __shared__ int meta_all[4][12];
-  const int lane = lane_id();
-  const int wave = threadIdx.x >> 6;
-  const int s = blockIdx.x * 4 + wave;
-  if (s >= shard_total) return;
-  // shard metadata is only defined once the frame's parse succeeded
-  const int frame = tiles[shards[s].tile].frame;
-  FrameState* st = &states[frame];
-  if (st->status != 0) return;
-  const DShard d = dshards[s];
-  const FrameDesc& f = frames[d.frame];
-  const uint32_t max_folded = (uint32_t)f.max_sample * 2u;
-  const uint8_t* body = f.stream + d.body_offset;
-  const int64_t body_length = d.body_length;
-  const int64_t n = d.count;
-  FT* out = folded + d.folded_offset;
-  const int mode = d.mode;
-
+__device__ __forceinline__ void dec_other_body(const uint8_t* body, int64_t body_length,
+                                               int64_t n, FT* out, int mode, uint32_t max_folded,
+                                               FrameState* st, int lane) {
   if (mode == 0) {  // decode.cu:237-295
-    if (lane != 0) return;
-    int64_t cursor = 0, produced = 0;
-    while (produced < n) {
-      uint32_t token = 0;
-      int shift = 0, used = 0;
-      for (;;) {
-        if (cursor >= body_length || used == 5) {
-          set_status(st, 1);
-          return;
+    if (lane == 0) {
+      int64_t cursor = 0, produced = 0;
+      int bad = 0;
+      while (produced < n && !bad) {
+        uint32_t token = 0;
+        int shift = 0, used = 0;
+        for (;;) {
+          if (cursor >= body_length || used == 5) {
+            bad = 1;
+            break;
+          }
+          const uint8_t byte = body[cursor++];
+          if (used == 4 && byte > 0x0f) {
+            bad = 1;
+            break;
+          }
+          token |= (uint32_t)(byte & 0x7f) << shift;
+          ++used;
+          if ((byte & 0x80) == 0) break;
+          shift += 7;
         }
-        const uint8_t byte = body[cursor++];
-        if (used == 4 && byte > 0x0f) {
-          set_status(st, 1);
-          return;
+        if (bad) break;
+        if (used != varint_len(token)) {
+          bad = 1;
+          break;
         }
-        token |= (uint32_t)(byte & 0x7f) << shift;
-        ++used;
-        if ((byte & 0x80) == 0) break;
-        shift += 7;
+        if ((token & 1) == 0) {
+          const int64_t run = token / 2 + 1;
+          if (run > n - produced) {
+            bad = 1;
+            break;
+          }
+          for (int64_t i = 0; i < run; ++i) out[produced++] = 0;
+        } else {
+          const uint32_t value = (token + 1) / 2;
+          if (value == 0 || value > max_folded) {
+            bad = 1;
+            break;
+          }
+          out[produced++] = (FT)value;
+        }
       }
-      if (used != varint_len(token)) {
-        set_status(st, 1);
-        return;
-      }
-      if ((token & 1) == 0) {
-        const int64_t run = token / 2 + 1;
-        if (run > n - produced) {
-          set_status(st, 1);
-          return;
-        }
-        for (int64_t i = 0; i < run; ++i) out[produced++] = 0;
-      } else {
-        const uint32_t value = (token + 1) / 2;
-        if (value == 0 || value > max_folded) {
-          set_status(st, 1);
-          return;
-        }
-        out[produced++] = (FT)value;
-      }
+      if (bad || cursor != body_length) set_status(st, 1);
     }
-    if (cursor != body_length) set_status(st, 1);
-    return;
-  }
-
-  if (mode >= 1 && mode <= 17) {  // decode.cu:297-340
+  } else if (mode >= 1 && mode <= 17) {  // decode.cu:297-340
     const int lane_count = (int)(n < 4 ? n : 4);
-    if (lane >= lane_count) return;
     const int64_t table_bytes = (int64_t)(lane_count - 1) * 4;
-    if (table_bytes > body_length) {
-      set_status(st, 2);
-      return;
-    }
-    int64_t lane_offset = table_bytes;
-    for (int i = 0; i < lane; ++i) lane_offset += rd_u32(body + i * 4);
-    const int64_t lane_length = lane + 1 < lane_count ? (int64_t)rd_u32(body + lane * 4)
-                                                      : body_length - lane_offset;
-    if (lane_offset > body_length || lane_length < 0 || lane_length > body_length - lane_offset) {
-      set_status(st, 2);
-      return;
-    }
-    BitReader reader{body + lane_offset, lane_length, 0, 0, 0, 0};
-    const int k = mode - 1;
-    for (int64_t i = lane; i < n; i += lane_count) {
-      uint32_t value = 0;
-      if (!reader.read(k, max_folded, &value)) {
-        set_status(st, 2);
-        return;
+    if (lane < lane_count) {
+      int ok = table_bytes <= body_length;
+      int64_t lane_offset = table_bytes;
+      for (int i = 0; ok && i < lane; ++i) lane_offset += rd_u32(body + i * 4);
+      const int64_t lane_length = !ok ? 0
+          : lane + 1 < lane_count ? (int64_t)rd_u32(body + lane * 4) : body_length - lane_offset;
+      if (ok && (lane_offset > body_length || lane_length < 0 ||
+                 lane_length > body_length - lane_offset)) {
+        ok = 0;
       }
-      out[i] = (FT)value;
+      if (ok) {
+        BitReader reader{body + lane_offset, lane_length, 0, 0, 0, 0};
+        const int k = mode - 1;
+        for (int64_t i = lane; i < n; i += lane_count) {
+          uint32_t value = 0;
+          if (!reader.read(k, max_folded, &value)) {
+            ok = 0;
+            break;
+          }
+          out[i] = (FT)value;
+        }
+        if (ok) {
+          const int64_t used_bytes = (reader.consumed + 7) / 8;
+          if (used_bytes != lane_length) ok = 0;
+          else if ((reader.consumed & 7) != 0) {
+            const uint8_t mask = (uint8_t)~((1u << (reader.consumed & 7)) - 1);
+            if ((body[lane_offset + used_bytes - 1] & mask) != 0) ok = 0;
+          }
+        }
+      }
+      if (!ok) set_status(st, 2);
     }
-    const int64_t used_bytes = (reader.consumed + 7) / 8;
-    if (used_bytes != lane_length) {
-      set_status(st, 2);
-      return;
-    }
-    if ((reader.consumed & 7) != 0) {
-      const uint8_t mask = (uint8_t)~((1u << (reader.consumed & 7)) - 1);
-      if ((body[lane_offset + used_bytes - 1] & mask) != 0) set_status(st, 2);
-    }
-    return;
-  }
-
-  if (mode == 18) {  // decode.cu:342-396
+  } else if (mode == 18) {  // decode.cu:342-396
     const int64_t block_count = (n + kBlockSymbols - 1) / kBlockSymbols;
     const int64_t block = lane;
-    if (block >= block_count) return;
-    int64_t cursor = 0;
-    for (int64_t prior = 0; prior < block; ++prior) {
-      if (cursor >= body_length) {
-        set_status(st, 3);
-        return;
+    if (block < block_count) {
+      int ok = 1;
+      int64_t cursor = 0;
+      for (int64_t prior = 0; ok && prior < block; ++prior) {
+        if (cursor >= body_length) {
+          ok = 0;
+          break;
+        }
+        const int w = body[cursor++];
+        const int64_t pr = n - prior * kBlockSymbols;
+        const int64_t pc = pr < kBlockSymbols ? pr : kBlockSymbols;
+        cursor += (pc * w + 7) / 8;
       }
-      const int w = body[cursor++];
-      const int64_t pr = n - prior * kBlockSymbols;
-      const int64_t pc = pr < kBlockSymbols ? pr : kBlockSymbols;
-      cursor += (pc * w + 7) / 8;
-    }
-    if (cursor >= body_length) {
-      set_status(st, 3);
-      return;
-    }
-    const int w = body[cursor++];
-    if (w > bit_length(max_folded)) {
-      set_status(st, 3);
-      return;
-    }
-    const int64_t rem = n - block * kBlockSymbols;
-    const int64_t cnt = rem < kBlockSymbols ? rem : kBlockSymbols;
-    const int64_t payload = (cnt * w + 7) / 8;
-    if (payload > body_length - cursor) {
-      set_status(st, 3);
-      return;
-    }
-    const uint8_t* packed = body + cursor;
-    int64_t bit = 0;
-    for (int64_t i = 0; i < cnt; ++i) {
-      uint32_t value = 0;
-      for (int b = 0; b < w; ++b, ++bit) {
-        value |= (uint32_t)((packed[bit >> 3] >> (bit & 7)) & 1) << b;
+      if (ok && cursor >= body_length) ok = 0;
+      const int w = ok ? body[cursor++] : 0;
+      if (ok && w > bit_length(max_folded)) ok = 0;
+      const int64_t rem = n - block * kBlockSymbols;
+      const int64_t cnt = rem < kBlockSymbols ? rem : kBlockSymbols;
+      const int64_t payload = (cnt * w + 7) / 8;
+      if (ok && payload > body_length - cursor) ok = 0;
+      if (ok) {
+        const uint8_t* packed = body + cursor;
+        int64_t bit = 0;
+        for (int64_t i = 0; i < cnt; ++i) {
+          uint32_t value = 0;
+          for (int bb = 0; bb < w; ++bb, ++bit) {
+            value |= (uint32_t)((packed[bit >> 3] >> (bit & 7)) & 1) << bb;
+          }
+          if (value > max_folded) {
+            ok = 0;
+            break;
+          }
+          out[block * kBlockSymbols + i] = (FT)value;
+        }
+        if (ok && block + 1 == block_count && cursor + payload != body_length) ok = 0;
+        if (ok && (bit & 7) != 0) {
+          const uint8_t mask = (uint8_t)~((1u << (bit & 7)) - 1);
+          if ((packed[payload - 1] & mask) != 0) ok = 0;
+        }
       }
-      if (value > max_folded) {
-        set_status(st, 3);
-        return;
-      }
-      out[block * kBlockSymbols + i] = (FT)value;
+      if (!ok) set_status(st, 3);
     }
-    if (block + 1 == block_count && cursor + payload != body_length) set_status(st, 3);
-    if ((bit & 7) != 0) {
-      const uint8_t mask = (uint8_t)~((1u << (bit & 7)) - 1);
-      if ((packed[payload - 1] & mask) != 0) set_status(st, 3);
-    }
-    return;
+  } else {
+    if (lane == 0) set_status(st, 4);
   }
+}
 
-  // order-0 rANS (decode.cu:441-549)
-  uint16_t* values = values_all[wave];
-  uint16_t* freq = freq_all[wave];
-  uint16_t* cum = cum_all[wave];
-  uint8_t* lut = lut_all[wave];
-  int* meta = meta_all[wave];  // [0]=valid [1]=n [2..5]=start [6..9]=len
-  __shared__ uint32_t st_all[4][4];
-  uint32_t* lane_state = st_all[wave];
-  if (lane == 0) {
+// Zero-run, Rice and block-pack shards (decode.cu:222-399): one wave per
+// shard, shards taken from the "other" list by a grid-stride loop.  The body
+// is first copied to LDS so the serial token walks and block-offset walks read
+// LDS instead of issuing dependent global loads.
+template <typename FT>
+__global__ __launch_bounds__(64) void k_dec_other(const FrameDesc* __restrict__ frames,
+                                                  FrameState* __restrict__ states,
+                                                  const DShard* __restrict__ dshards,
+                                                  const int32_t* __restrict__ lists,
+                                                  const int32_t* __restrict__ list_count,
+                                                  int shard_total, FT* __restrict__ folded,
+                                                  char* gbase) {
+  __shared__ __attribute__((aligned(16))) uint8_t body_lds[65536 / 8 + 768 + 16];
+  const int lane = lane_id();
+  const int total = list_count[1];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int s = lists[shard_total + item];
+    const DShard d = dshards[s];
+    FrameState* st = &states[d.frame];
+    if (st->status != 0) continue;  // uniform
+    const FrameDesc& f = frames[d.frame];
+    const uint32_t max_folded = (uint32_t)f.max_sample * 2u;
+    const uint8_t* gbody = as_global(gbase, (const uint8_t*)f.stream + d.body_offset);
+    const int64_t body_length = d.body_length;
+    const int64_t n = d.count;
+    FT* out = folded + d.folded_offset;
+    const int mode = d.mode;
+    // bodies chosen by the encoder are <= 8736 bytes; longer (malformed but
+    // parseable) bodies are read from global memory directly.  Staging copies
+    // the aligned 16-byte chunks that hold the body: such a chunk never
+    // crosses a page, so the bytes before and after the body it covers are
+    // readable whenever the body is.
+    const int head = (int)((uintptr_t)gbody & 15);
+    const bool staged = body_length + head <= (int64_t)sizeof(body_lds);
+    if (staged) {
+      const uint4* chunks = (const uint4*)(gbody - head);
+      const int nchunks = body_length > 0 ? (int)((head + body_length + 15) >> 4) : 0;
+      for (int k = lane; k < nchunks; k += 64) ((uint4*)body_lds)[k] = chunks[k];
+    }
+    __syncthreads();
+    if (staged) {
+      dec_other_body(body_lds + head, body_length, n, out, mode, max_folded, st, lane);
+    } else {
+      dec_other_body(gbody, body_length, n, out, mode, max_folded, st, lane);
+    }
+    __syncthreads();
+  }
+}
+
+// Order-0 rANS shards (decode.cu:441-549): four shards per wave, 16 lanes per
+// shard.  Per shard: the table bytes are staged in LDS and parsed by one lane
+// with the reference's checks; the 16 lanes fill the 4096-slot rank LUT; four
+// lanes run the rANS lanes, each with an 8-byte lookahead of its byte stream
+// so the state recurrence only waits on two LDS lookups per symbol.
+constexpr int kO0Groups = 4;
+constexpr int kO0Stage = 1312;  // >= any valid table + lane lengths + states
+
+struct O0Meta {
+  int valid;
+  int n;
+  int start[4];
+  int length[4];
+  uint32_t state[4];
+};
+
+template <typename FT>
+__global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ frames,
+                                               FrameState* __restrict__ states,
+                                               const DShard* __restrict__ dshards,
+                                               const int32_t* __restrict__ lists,
+                                               const int32_t* __restrict__ list_count,
+                                               FT* __restrict__ folded, char* gbase) {
+  __shared__ __attribute__((aligned(16))) uint8_t lut_all[kO0Groups][4096];
+  __shared__ uint32_t ent_all[kO0Groups][256];
+  __shared__ uint16_t val_all[kO0Groups][256];
+  __shared__ O0Meta meta_all[kO0Groups];
+  const int lane = lane_id();
+  const int g = lane >> 4;
+  const int i = lane & 15;
+  const int total = list_count[0];
+  const int item = blockIdx.x * kO0Groups + g;
+  const bool has = item < total;
+  uint8_t* lut = lut_all[g];
+  uint32_t* ent = ent_all[g];
+  uint16_t* val = val_all[g];
+  O0Meta& meta = meta_all[g];
+  DShard d = {};
+  FrameState* st = nullptr;
+  const FrameDesc* f = nullptr;
+  bool go = false;
+  if (has) {
+    d = dshards[lists[item]];
+    st = &states[d.frame];
+    f = &frames[d.frame];
+    go = st->status == 0;
+  }
+  const uint8_t* body =
+      as_global(gbase, go ? (const uint8_t*)f->stream + d.body_offset : (const uint8_t*)gbase);
+  const int64_t body_length = go ? d.body_length : 0;
+  // stage the table bytes (aliasing the LUT) as the aligned 16-byte chunks
+  // that hold them; such a chunk never crosses a page
+  const int head = (int)((uintptr_t)body & 15);
+  const int nstage = (int)min(body_length, (int64_t)kO0Stage);
+  const int nchunks = nstage > 0 ? (head + nstage + 15) >> 4 : 0;
+  const uint4* chunks = (const uint4*)(body - head);
+  for (int k = i; k < nchunks; k += 16) ((uint4*)lut)[k] = chunks[k];
+  __syncthreads();
+  if (go && i == 0) {
+    const uint32_t max_folded = (uint32_t)f->max_sample * 2u;
+    const uint8_t* b = lut + head;
+    // reads past the stage only happen for tables no valid stream can have
+    const int64_t limit = min(body_length, (int64_t)kO0Stage);
     int valid = 1;
     int64_t cursor = 0;
-    if (body_length < 1 || body[cursor++] != kRansTableLog) valid = 0;
+    if (body_length < 1 || b[cursor++] != kRansTableLog) valid = 0;
     uint32_t nsym = 0;
-    if (valid && !read_varint(body, body_length, &cursor, &nsym)) valid = 0;
+    if (valid && !read_varint(b, limit, &cursor, &nsym)) valid = 0;
     if (nsym == 0 || nsym > 255) valid = 0;
     uint32_t previous = 0, assigned = 0;
-    for (uint32_t i = 0; valid && i < nsym; ++i) {
+    for (uint32_t k = 0; valid && k < nsym; ++k) {
       uint32_t delta = 0;
-      if (!read_varint(body, body_length, &cursor, &delta) || (i != 0 && delta == 0) ||
+      if (!read_varint(b, limit, &cursor, &delta) || (k != 0 && delta == 0) ||
           previous + delta > max_folded) {
         valid = 0;
         break;
       }
       const uint32_t value = previous + delta;
       uint32_t fr = (1u << kRansTableLog) - assigned;
-      if (i + 1 != nsym && !read_varint(body, body_length, &cursor, &fr)) {
+      if (k + 1 != nsym && !read_varint(b, limit, &cursor, &fr)) {
         valid = 0;
         break;
       }
@@ -391,32 +486,32 @@ __global__ __launch_bounds__(256) void k_entropy(const FrameDesc* __restrict__ f
         break;
       }
       if (fr == 0 || assigned + fr > (1u << kRansTableLog) ||
-          (i + 1 != nsym && assigned + fr == (1u << kRansTableLog))) {
+          (k + 1 != nsym && assigned + fr == (1u << kRansTableLog))) {
         valid = 0;
         break;
       }
-      values[i] = (uint16_t)value;
-      freq[i] = (uint16_t)fr;
-      cum[i] = (uint16_t)assigned;
+      val[k] = (uint16_t)value;
+      ent[k] = (fr & 0xffffu) | ((uint32_t)assigned << 16);
       assigned += fr;
       previous = value;
     }
     if (assigned != (1u << kRansTableLog)) valid = 0;
     int32_t total_payload = 0;
-    int32_t len[4] = {0, 0, 0, 0};
+    int* len = meta.length;  // LDS: indexed by a loop counter
+    for (int l = 0; l < 4; ++l) len[l] = 0;
     for (int l = 0; valid && l < 3; ++l) {
-      if (cursor + 4 > body_length) {
+      if (cursor + 4 > limit) {
         valid = 0;
       } else {
-        len[l] = (int32_t)rd_u32(body + cursor);
+        len[l] = (int32_t)rd_u32(b + cursor);
         total_payload = (int32_t)((uint32_t)total_payload + (uint32_t)len[l]);
         cursor += 4;
       }
     }
-    if (valid && cursor + 16 > body_length) valid = 0;
+    if (valid && cursor + 16 > limit) valid = 0;
     for (int l = 0; valid && l < 4; ++l) {
-      lane_state[l] = rd_u32(body + cursor + l * 4);
-      if (lane_state[l] < kRansL) valid = 0;
+      meta.state[l] = rd_u32(b + cursor + l * 4);
+      if (meta.state[l] < kRansL) valid = 0;
     }
     if (valid) {
       cursor += 16;
@@ -427,44 +522,105 @@ __global__ __launch_bounds__(256) void k_entropy(const FrameDesc* __restrict__ f
       }
       int64_t c = cursor;
       for (int l = 0; l < 4; ++l) {
-        meta[2 + l] = (int)c;
-        meta[6 + l] = len[l];
+        meta.start[l] = (int)c;
         c += len[l];
       }
       if (c != body_length) valid = 0;
     }
-    meta[0] = valid;
-    meta[1] = (int)nsym;
+    meta.valid = valid;
+    meta.n = (int)nsym;
     if (!valid) set_status(st, 5);
   }
-  __builtin_amdgcn_wave_barrier();
-  if (!meta[0]) return;
-  const int nsym = meta[1];
-  for (int sym = 0; sym < nsym; ++sym) {
-    const int b = cum[sym];
-    const int e = b + freq[sym];
-    for (int slot = b + lane; slot < e; slot += 64) lut[slot] = (uint8_t)sym;
-  }
-  __builtin_amdgcn_wave_barrier();
-  if (lane >= 4) return;
-  uint32_t x = lane_state[lane];
-  const int start = meta[2 + lane];
-  const int length = meta[6 + lane];
-  int read = 0;
-  for (int64_t i = lane; i < n; i += 4) {
-    const uint32_t slot = x & ((1u << kRansTableLog) - 1);
-    const int sym = lut[slot];
-    out[i] = (FT)values[sym];
-    x = (uint32_t)freq[sym] * (x >> kRansTableLog) + slot - cum[sym];
-    while (x < kRansL) {
-      if (read >= length) {
-        set_status(st, 5);
-        return;
+  __syncthreads();
+  const bool ok = go && meta.valid;
+  // rank LUT: lane i fills slots [256 i, 256 i + 256), four per store
+  if (ok) {
+    const int nsym = meta.n;
+    const int lo_slot = i * 256;
+    int rk = 0;
+    int next_cum = nsym > 1 ? (int)(ent[1] >> 16) : (1 << kRansTableLog);
+    while (next_cum <= lo_slot) {
+      ++rk;
+      next_cum = rk + 1 < nsym ? (int)(ent[rk + 1] >> 16) : (1 << kRansTableLog);
+    }
+    uint32_t* lut_w = (uint32_t*)lut + i * 64;
+    for (int w = 0; w < 64; ++w) {
+      uint32_t word = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int slot = lo_slot + w * 4 + q;
+        while (next_cum <= slot) {
+          ++rk;
+          next_cum = rk + 1 < nsym ? (int)(ent[rk + 1] >> 16) : (1 << kRansTableLog);
+        }
+        word |= (uint32_t)rk << (8 * q);
       }
-      x = (x << 8) | body[start + read++];
+      lut_w[w] = word;
     }
   }
-  if (read != length || x != kRansL) set_status(st, 5);
+  __syncthreads();
+  if (!ok || i >= 4) return;
+  // rANS lane i.  Its bytes are read as aligned 16-byte chunks: `cur` is the
+  // chunk being consumed (next byte at `pos`), `nxt` the following chunk,
+  // requested as soon as `cur` becomes current.  Only chunks holding at least
+  // one byte of the lane are read.  The window `bb` holds `avail` bytes, most
+  // significant first, and is topped up whenever it falls to four bytes, so
+  // the (at most two) renormalisation reads of a step never wait on memory.
+  const int r = i;
+  const uint8_t* src = body + meta.start[r];
+  const int length = meta.length[r];
+  uint32_t x = meta.state[r];
+  FT* out = folded + d.folded_offset;
+  const int n = d.count;
+  int pos = (int)((uintptr_t)src & 15);
+  const uint4* cp = (const uint4*)(src - pos);
+  int left = length;  // lane bytes not yet moved into the window
+  int c = 0;
+  uint4 cur = make_uint4(0, 0, 0, 0);
+  uint4 nxt = cur;
+  if (left > 0) cur = cp[0];
+  if (left > 16 - pos) nxt = cp[1];
+  uint64_t bb = 0;
+  int avail = 0;
+  int bad = 0;
+  for (int idx = r; idx < n; idx += 4) {
+    while (avail <= 4 && left > 0) {
+      // dword pos/4 of the chunk, by shifts
+      const uint64_t half = (pos & 8) ? (((uint64_t)cur.w << 32) | cur.z)
+                                      : (((uint64_t)cur.y << 32) | cur.x);
+      const uint32_t w = (uint32_t)(half >> ((pos & 4) << 3));
+      const int skip = pos & 3;
+      const int take = min(4 - skip, left);
+      // byte `pos` to the top, later bytes below it
+      const uint32_t msb_first = __builtin_bswap32(w) << (8 * skip);
+      bb |= ((uint64_t)msb_first << 32) >> (8 * avail);
+      avail += take;
+      left -= take;
+      pos += take;
+      if (pos == 16) {
+        ++c;
+        pos = 0;
+        cur = nxt;
+        if (left > 16) nxt = cp[c + 1];
+      }
+    }
+    const uint32_t slot = x & ((1u << kRansTableLog) - 1);
+    const int sym = lut[slot];
+    const uint32_t e = ent[sym];
+    out[idx] = (FT)val[sym];
+    x = __umul24(e & 0xffffu, x >> kRansTableLog) + slot - (e >> 16);
+    while (x < kRansL) {
+      if (avail == 0) {
+        bad = 1;
+        break;
+      }
+      x = (x << 8) | (uint32_t)(bb >> 56);
+      bb <<= 8;
+      --avail;
+    }
+    if (bad) break;
+  }
+  if (bad || left != 0 || avail != 0 || x != kRansL) set_status(st, 5);
 }
 
 // predictor="serial": one thread per tile in raster order (decode.cu:608-656).
@@ -514,28 +670,35 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
   const dim3 fgrid((b.frame_count + 63) / 64);
   {
     FVH_TIMED("k_dstate_init", stream);
-    hipLaunchKernelGGL(k_dstate_init, fgrid, dim3(64), 0, stream, b.states, b.frame_count);
+    hipLaunchKernelGGL(k_dstate_init, fgrid, dim3(64), 0, stream, b.states, b.frame_count,
+                       b.list_count);
   }
   launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.tile_total, stream);
   {
     FVH_TIMED("k_parse", stream);
     hipLaunchKernelGGL(k_parse, dim3((b.tile_total + 255) / 256), dim3(256), 0, stream, b.frames,
-                       b.states, b.tiles, b.tile_total, b.dshards);
+                       b.states, b.tiles, b.tile_total, b.dshards, b.lists, b.list_count,
+                       b.shard_total, (char*)b.folded);
   }
   {
-    FVH_TIMED("k_entropy", stream);
-    hipLaunchKernelGGL((k_entropy<FT>), dim3((b.shard_total + 3) / 4), dim3(256), 0, stream,
-                       b.frames, b.states, b.tiles, b.shards, b.dshards, b.shard_total,
-                       (FT*)b.folded);
+    FVH_TIMED("k_dec_o0", stream);
+    hipLaunchKernelGGL((k_dec_o0<FT>), dim3((b.shard_total + kO0Groups - 1) / kO0Groups), dim3(64),
+                       0, stream, b.frames, b.states, b.dshards, b.lists, b.list_count,
+                       (FT*)b.folded, (char*)b.folded);
+  }
+  {
+    FVH_TIMED("k_dec_other", stream);
+    const int grid = min(b.shard_total, 4096);
+    hipLaunchKernelGGL((k_dec_other<FT>), dim3(grid), dim3(64), 0, stream, b.frames, b.states,
+                       b.dshards, b.lists, b.list_count, b.shard_total, (FT*)b.folded,
+                       (char*)b.folded);
   }
   if (b.wavefront) {
     launch_reconstruct<FT>(b, stream);
   } else {
-    {
-      FVH_TIMED("k_serial", stream);
-      hipLaunchKernelGGL((k_serial<FT>), dim3((b.tile_total + 255) / 256), dim3(256), 0, stream,
-                         b.frames, b.states, b.tiles, b.tile_total, (const FT*)b.folded);
-    }
+    FVH_TIMED("k_serial", stream);
+    hipLaunchKernelGGL((k_serial<FT>), dim3((b.tile_total + 255) / 256), dim3(256), 0, stream,
+                       b.frames, b.states, b.tiles, b.tile_total, (const FT*)b.folded);
   }
   {
     FVH_TIMED("k_dresults", stream);

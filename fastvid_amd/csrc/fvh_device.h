@@ -80,3 +80,18 @@ __device__ __forceinline__ uint32_t quant_magic(int32_t d) {
 }
 
 }  // namespace fvh
+
+namespace fvh {
+
+// Pointers read out of descriptors are generic (flat) to the compiler, which
+// tracks flat accesses with conservative counters and drains every
+// outstanding load before the next one.
+// Re-derive `p` from `gbase`, a pointer kernel argument: same address, but the
+// compiler now knows the access is to global memory (kernel arguments are in
+// the global address space) and keeps loads in flight with per-load counters.
+template <typename T>
+__device__ __forceinline__ T* as_global(char* gbase, T* p) {
+  return (T*)(gbase + ((const char*)p - (const char*)gbase));
+}
+
+}  // namespace fvh

@@ -128,7 +128,8 @@ template <bool kEncode, typename FT>
 __global__ __launch_bounds__(256) void k_wavefront(const FrameDesc* __restrict__ frames,
                                                    FrameState* __restrict__ states,
                                                    const TileDesc* __restrict__ tiles,
-                                                   int tile_total, FT* __restrict__ folded) {
+                                                   int tile_total, FT* __restrict__ folded,
+                                                   char* gbase) {
   __shared__ uint16_t ring_all[4][kRingMax];
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
@@ -143,7 +144,9 @@ __global__ __launch_bounds__(256) void k_wavefront(const FrameDesc* __restrict__
     step = st->step;
     med = st->use_med;
     if (step == 1) return;  // lossless: k_lossless
+    if (f.band) return;     // k_band
   } else {
+    if (f.band) return;     // k_band
     if (st->status != 0) return;
     step = f.step;
     med = f.use_med;
@@ -163,10 +166,10 @@ __global__ __launch_bounds__(256) void k_wavefront(const FrameDesc* __restrict__
   uint16_t* dst = nullptr;
   int64_t dst_pitch = 0;
   if (kEncode) {
-    src = f.plane[td.plane] + td.src_offset;
+    src = as_global(gbase, f.plane[td.plane] + td.src_offset);
     src_pitch = f.pitch[td.plane];
   } else {
-    dst = f.out_plane[td.plane] + td.dst_offset;
+    dst = as_global(gbase, f.out_plane[td.plane] + td.dst_offset);
     dst_pitch = f.plane_width[td.plane];
   }
   int b = 0, x = -lane;
@@ -217,7 +220,7 @@ template <typename FT>
 __global__ __launch_bounds__(256) void k_lossless(const FrameDesc* __restrict__ frames,
                                                   FrameState* __restrict__ states,
                                                   const TileDesc* __restrict__ tiles,
-                                                  FT* __restrict__ folded) {
+                                                  FT* __restrict__ folded, char* gbase) {
   const int t = blockIdx.x;
   const TileDesc td = tiles[t];
   FrameState* st = &states[td.frame];
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(256) void k_lossless(const FrameDesc* __restrict__ 
   const FrameDesc& f = frames[td.frame];
   const int med = st->use_med;
   const int max_sample = f.max_sample;
-  const uint16_t* src = f.plane[td.plane] + td.src_offset;
+  const uint16_t* src = as_global(gbase, f.plane[td.plane] + td.src_offset);
   const int64_t pitch = f.pitch[td.plane];
   FT* fold = folded + td.folded_base;
   const int W = td.width, H = td.height;
@@ -358,18 +361,19 @@ __device__ __forceinline__ ClampFn norm_fn(uint32_t prefix, int count, int remai
   return ClampFn{1, e, u};
 }
 
-// rANS encoder entry of one symbol: freq (13 bits) | cum << 13 (12 bits) |
-// c << 25 (4 bits) | m << 32, with c = ceil(log2 freq) and
-// m = ceil(2^(31+c) / freq): floor(x / freq) = mul_hi(x, m) >> (c-1) for every
-// x < 2^31 (freq >= 2; freq == 1 divides trivially).
-__device__ __forceinline__ uint64_t rans_entry(uint32_t freq, uint32_t cum) {
+// rANS encoder entry of one symbol, indexed by its rank among the shard's
+// present values: x = freq (13 bits) | cum << 13 (12 bits) | c << 25 (4 bits),
+// y = m, with c = ceil(log2 freq) and m = ceil(2^(31+c) / freq):
+// floor(x / freq) = mul_hi(x, m) >> (c-1) for every state x < 2^31 (freq >= 2;
+// freq == 1 divides trivially).
+__device__ __forceinline__ uint2 rans_entry(uint32_t freq, uint32_t cum) {
   uint32_t c = 0;
   uint32_t m = 0;
   if (freq >= 2) {
     c = 32 - __builtin_clz(freq - 1);
     m = (uint32_t)(((1ull << (31 + c)) + freq - 1) / freq);
   }
-  return (uint64_t)freq | ((uint64_t)cum << 13) | ((uint64_t)c << 25) | ((uint64_t)m << 32);
+  return make_uint2(freq | (cum << 13) | (c << 25), m);
 }
 
 __device__ __forceinline__ int write_varint_bytes(uint8_t* out, uint32_t value) {
@@ -396,6 +400,7 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
                                                  ShardRec* __restrict__ recs,
                                                  uint8_t* __restrict__ scratch) {
   __shared__ uint32_t hist_all[4][512];
+  __shared__ __attribute__((aligned(16))) uint8_t ranks_all[4][kShardSymbols];
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   const int s = blockIdx.x * 4 + wave;
@@ -502,8 +507,9 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
       }
       int before = shfl_up(prev_val, 1);
       if (lane == 0 || before < 0) before = 0;
-      // frequencies, encoder entries and sparse-table entry sizes
-      uint64_t* tab = (uint64_t*)(sbase + kScratchTable);
+      // frequencies, encoder entries (by rank), value -> rank map in the
+      // histogram's LDS, sparse-table entry sizes
+      uint2* tab = (uint2*)(sbase + kScratchTable);
       uint32_t freqs[8];
       int lens = 0;
       {
@@ -518,7 +524,8 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
             const int rem = distinct - rank - 1;
             const int end = clamp_apply(norm_fn(p, count, rem), assigned);
             freqs[k] = (uint32_t)(end - assigned);
-            tab[lane * 8 + k] = rans_entry(freqs[k], (uint32_t)assigned);
+            tab[rank] = rans_entry(freqs[k], (uint32_t)assigned);
+            hist[lane * 8 + k] = (uint32_t)rank;
             lens += varint_len((uint32_t)(lane * 8 + k - pv)) + (rem ? varint_len(freqs[k]) : 0);
             assigned = end;
             pv = lane * 8 + k;
@@ -551,6 +558,19 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
       rec.sparse_len = (uint16_t)(hdr + shfl(len_incl, 63));
       rec.distinct = (uint16_t)distinct;
       rec.supported = 1;
+      // rank of every symbol, staged in LDS and written with 16-byte stores
+      // (the rANS lanes read it instead of the residuals)
+      __builtin_amdgcn_wave_barrier();
+      uint8_t* stage = ranks_all[wave];
+#pragma unroll
+      for (int j = 0; j < 64; ++j) {
+        const int i = (j << 6) + lane;
+        if (i < count) stage[i] = (uint8_t)hist[val[j]];
+      }
+      __builtin_amdgcn_wave_barrier();
+      uint4* dst = (uint4*)(sbase + kScratchRanks);
+      const uint4* srcv = (const uint4*)stage;
+      for (int w = lane; w < (count + 15) / 16; w += 64) dst[w] = srcv[w];
     }
   }
 
@@ -645,62 +665,126 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
 }
 
 // Order-0 rANS encode (encode.cu:398-418): one thread per rANS lane, four
-// consecutive threads per shard, 16 shards per wave.  Division by the symbol
-// frequency uses the per-symbol reciprocal from k_analyze.  Emitted bytes are
-// stored backwards from the end of the lane's scratch area, so the area ends
-// with the lane's bytes already in decode order.  The last step decides the
-// shard's mode: order-0 when forced or strictly smaller than the best of
-// three (encode.cu:423-429).
+// consecutive threads per shard, 16 shards per wave (one wave per block).
+// The shard's rank-indexed entries live in LDS; each lane walks its symbols
+// backwards reading one rank byte per symbol from the rank stream, loaded two
+// 8-symbol chunks ahead so the state recurrence never waits on memory.
+// Division by the symbol frequency is the reciprocal multiply of rans_entry.
+// Emitted bytes go backwards from the end of the lane's scratch area, packed
+// four per store, so the area ends with the lane's bytes in decode order.
+// The last step decides the shard's mode: order-0 when forced or strictly
+// smaller than the best of three (encode.cu:423-429).
+constexpr int kRansShards = 16;
+
 template <typename FT>
-__global__ __launch_bounds__(256) void k_rans(const FrameDesc* __restrict__ frames,
-                                              FrameState* __restrict__ states,
-                                              const TileDesc* __restrict__ tiles,
-                                              const ShardDesc* __restrict__ shards,
-                                              int shard_total, const FT* __restrict__ folded,
-                                              ShardRec* __restrict__ recs,
-                                              uint8_t* __restrict__ scratch) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = gid >> 2;
-  const int r = gid & 3;
-  if (s >= shard_total) return;
-  const ShardDesc sd = shards[s];
-  const TileDesc td = tiles[sd.tile];
-  FrameState* st = &states[td.frame];
-  const bool act = st->active && recs[s].supported;
-  int nbytes = 0;
-  if (act) {
+__global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frames,
+                                             FrameState* __restrict__ states,
+                                             const TileDesc* __restrict__ tiles,
+                                             const ShardDesc* __restrict__ shards,
+                                             int shard_total, const FT* __restrict__ folded,
+                                             ShardRec* __restrict__ recs,
+                                             uint8_t* __restrict__ scratch) {
+  (void)folded;
+  __shared__ uint2 ent_all[kRansShards][256];
+  const int lane = lane_id();
+  const int g = lane >> 2;
+  const int r = lane & 3;
+  const int s = blockIdx.x * kRansShards + g;
+  bool act = false;
+  int count = 0, distinct = 0;
+  TileDesc td = {};
+  FrameState* st = nullptr;
+  if (s < shard_total) {
+    const ShardDesc sd = shards[s];
+    td = tiles[sd.tile];
+    st = &states[td.frame];
+    act = st->active && recs[s].supported;
     const int64_t local0 = (int64_t)sd.local * kShardSymbols;
-    const int count = (int)min((int64_t)kShardSymbols, (int64_t)td.width * td.height - local0);
-    const FT* v = folded + td.folded_base + local0;
-    uint8_t* sbase = scratch + (int64_t)s * kScratchBytes;
-    const uint64_t* tab = (const uint64_t*)(sbase + kScratchTable);
+    count = (int)min((int64_t)kShardSymbols, (int64_t)td.width * td.height - local0);
+    distinct = recs[s].distinct;
+  }
+  uint8_t* sbase = scratch + (int64_t)(act ? s : 0) * kScratchBytes;
+  uint2* ent = ent_all[g];
+  if (act) {
+    const uint2* tab = (const uint2*)(sbase + kScratchTable);
+    for (int k = r; k < distinct; k += 4) ent[k] = tab[k];
+  }
+  __syncthreads();
+  int nbytes = 0;
+  uint32_t x = kRansL;
+  if (act) {
+    const uint32_t* rw = (const uint32_t*)(sbase + kScratchRanks);
     uint8_t* lane_end = sbase + kScratchLanes + (r + 1) * kLaneBytes;
-    uint32_t x = kRansL;
-    for (int i = count - 1 - ((count - 1 - r) & 3); i >= 0; i -= 4) {
-      const uint64_t e = tab[v[i]];
-      const uint32_t lo = (uint32_t)e;
-      const uint32_t freq = lo & 0x1fffu;
-      const uint32_t cum = (lo >> 13) & 0xfffu;
-      const uint32_t c = (lo >> 25) & 0xfu;
-      const uint32_t m = (uint32_t)(e >> 32);
-      const uint32_t lim = freq << 19;
-      if (x >= lim) {
-        if (nbytes < kLaneBytes) lane_end[-1 - nbytes] = (uint8_t)x;
-        ++nbytes;
-        x >>= 8;
-        if (x >= lim) {
-          if (nbytes < kLaneBytes) lane_end[-1 - nbytes] = (uint8_t)x;
-          ++nbytes;
-          x >>= 8;
+    uint32_t* lane_end_w = (uint32_t*)lane_end;
+    const int start = count - 1 - ((count - 1 - r) & 3);
+    const int nsym = start >= 0 ? (start >> 2) + 1 : 0;  // symbols of this lane
+    const uint32_t shift = 8u * (uint32_t)r;
+    uint32_t acc = 0;      // emitted bytes not yet stored, most recent in the low byte
+    int pending = 0;
+    int stored_words = 0;
+    // ranks of symbols start, start-4, ... : word (start-4k)>>2 = (start>>2)-k
+    const int w0 = start >> 2;
+    uint32_t a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = u < nsym ? rw[w0 - u] : 0u;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) b[u] = 8 + u < nsym ? rw[w0 - 8 - u] : 0u;
+    for (int k0 = 0; k0 < nsym; k0 += 8) {
+      uint32_t cur[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        cur[u] = a[u];
+        a[u] = b[u];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + 16 + u;
+        b[u] = k < nsym ? rw[w0 - k] : 0u;
+      }
+      uint2 e[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) e[u] = ent[(cur[u] >> shift) & 0xffu];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (k0 + u < nsym) {
+          const uint32_t freq = e[u].x & 0x1fffu;
+          const uint32_t cum = (e[u].x >> 13) & 0xfffu;
+          const uint32_t c = e[u].x >> 25;
+          const uint32_t lim = freq << 19;
+          if (x >= lim) {
+            acc = (acc << 8) | (x & 0xffu);
+            x >>= 8;
+            if (++pending == 4) {
+              if (stored_words < kLaneBytes / 4) lane_end_w[-1 - stored_words] = acc;
+              ++stored_words;
+              pending = 0;
+            }
+            if (x >= lim) {
+              acc = (acc << 8) | (x & 0xffu);
+              x >>= 8;
+              if (++pending == 4) {
+                if (stored_words < kLaneBytes / 4) lane_end_w[-1 - stored_words] = acc;
+                ++stored_words;
+                pending = 0;
+              }
+            }
+          }
+          const uint32_t q = c ? (__umulhi(x, e[u].y) >> (c - 1)) : x;
+          x += __umul24(q, 4096u - freq) + cum;
         }
       }
-      const uint32_t q = c ? (__umulhi(x, m) >> (c - 1)) : x;
-      x += q * (4096u - freq) + cum;
+    }
+    // the k-th emitted byte sits at lane_end - k; the last `pending` bytes are
+    // the low bytes of acc, most recent lowest
+    nbytes = stored_words * 4 + pending;
+    for (int j = 1; j <= pending; ++j) {
+      const int pos = stored_words * 4 + j;
+      if (pos <= kLaneBytes) lane_end[-pos] = (uint8_t)(acc >> (8 * (pending - j)));
     }
     if (nbytes > kLaneBytes) set_status(st, kStatusLaneOverflow);
     recs[s].states[r] = x;
   }
-  const int base = lane_id() & ~3;
+  const int base = lane & ~3;
   const int b0 = shfl(nbytes, base);
   const int b1 = shfl(nbytes, base + 1);
   const int b2 = shfl(nbytes, base + 2);
@@ -755,7 +839,8 @@ __global__ __launch_bounds__(256) void k_frame_finish(const FrameDesc* __restric
                                                       const TileDesc* __restrict__ tiles,
                                                       const int64_t* __restrict__ tile_len,
                                                       int64_t* __restrict__ tile_off,
-                                                      Results* __restrict__ results, int final) {
+                                                      Results* __restrict__ results, int final,
+                                                      char* gbase) {
   __shared__ int64_t part[256];
   __shared__ int too_large;
   const FrameDesc& f = frames[blockIdx.x];
@@ -794,7 +879,9 @@ __global__ __launch_bounds__(256) void k_frame_finish(const FrameDesc* __restric
         w[5] = (uint32_t)(uint64_t)off;
         w[6] = (uint32_t)((uint64_t)off >> 32);
         w[7] = (uint32_t)len;
-        if (st->status == 0) put_words(f.stream + kHeaderBytes + (int64_t)i * kDirectoryEntryBytes, w, 8);
+        if (st->status == 0) {
+          put_words(as_global(gbase, f.stream + kHeaderBytes + (int64_t)i * kDirectoryEntryBytes), w, 8);
+        }
       }
     }
     running += part[255];
@@ -818,7 +905,7 @@ __global__ __launch_bounds__(256) void k_frame_finish(const FrameDesc* __restric
         w[5] = f.fps_num;
         w[6] = f.fps_den;
         w[7] = (uint32_t)T;
-        put_words(f.stream, w, 8);
+        put_words(as_global(gbase, f.stream), w, 8);
       }
       st->stream_length = length;
       Results res;
@@ -852,7 +939,7 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
                                               int shard_total, const FT* __restrict__ folded,
                                               const ShardRec* __restrict__ recs,
                                               const uint8_t* __restrict__ scratch,
-                                              const int64_t* __restrict__ tile_off) {
+                                              const int64_t* __restrict__ tile_off, char* gbase) {
   __shared__ uint32_t buf_all[4][kBufWords];
   __shared__ uint64_t zm_all[4][65];
   const int lane = lane_id();
@@ -867,7 +954,7 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
   const int64_t local0 = (int64_t)sd.local * kShardSymbols;
   const int count = (int)min((int64_t)kShardSymbols, (int64_t)td.width * td.height - local0);
   const FT* v = folded + td.folded_base + local0;
-  uint8_t* out = f.stream + tile_off[sd.tile] + rec.offset;
+  uint8_t* out = as_global(gbase, f.stream + tile_off[sd.tile] + rec.offset);
   const int total = 3 + (int)rec.body;
   uint32_t* buf = buf_all[wave];
   uint8_t* bytes = (uint8_t*)buf;
@@ -1038,18 +1125,22 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
 template <typename FT>
 static void encode_pass(const EncodeBuffers& b, hipStream_t stream) {
   FT* folded = (FT*)b.folded;
-  if (b.any_lossy) {
+  if (b.any_lossy && b.any_band) {
+    launch_band<true, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.any_med,
+                          b.any_grad, stream);
+  }
+  if (b.any_lossy && b.any_nonband) {
     {
       FVH_TIMED("k_wavefront_enc", stream);
       hipLaunchKernelGGL((k_wavefront<true, FT>), dim3((b.tile_total + 3) / 4), dim3(256), 0, stream,
-                         b.frames, b.states, b.tiles, b.tile_total, folded);
+                         b.frames, b.states, b.tiles, b.tile_total, folded, (char*)folded);
     }
   }
   if (b.any_lossless) {
     {
       FVH_TIMED("k_lossless", stream);
       hipLaunchKernelGGL((k_lossless<FT>), dim3(b.tile_total), dim3(256), 0, stream, b.frames,
-                         b.states, b.tiles, folded);
+                         b.states, b.tiles, folded, (char*)folded);
     }
   }
   if (b.any_wide) {
@@ -1067,7 +1158,7 @@ static void encode_pass(const EncodeBuffers& b, hipStream_t stream) {
   }
   {
     FVH_TIMED("k_rans", stream);
-    hipLaunchKernelGGL((k_rans<FT>), dim3((b.shard_total * 4 + 255) / 256), dim3(256), 0, stream,
+    hipLaunchKernelGGL((k_rans<FT>), dim3((b.shard_total + kRansShards - 1) / kRansShards), dim3(64), 0, stream,
                        b.frames, b.states, b.tiles, b.shards, b.shard_total, folded, b.recs,
                        b.scratch);
   }
@@ -1091,7 +1182,7 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
     {
       FVH_TIMED("k_frame_finish", stream);
       hipLaunchKernelGGL(k_frame_finish, dim3(b.frame_count), dim3(256), 0, stream, b.frames,
-                         b.states, b.tiles, b.tile_len, b.tile_off, b.results, 0);
+                         b.states, b.tiles, b.tile_len, b.tile_off, b.results, 0, (char*)b.folded);
     }
     {
       FVH_TIMED("k_decide", stream);
@@ -1107,7 +1198,7 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
       {
         FVH_TIMED("k_frame_finish", stream);
         hipLaunchKernelGGL(k_frame_finish, dim3(b.frame_count), dim3(256), 0, stream, b.frames,
-                           b.states, b.tiles, b.tile_len, b.tile_off, b.results, 0);
+                           b.states, b.tiles, b.tile_len, b.tile_off, b.results, 0, (char*)b.folded);
       }
       {
         FVH_TIMED("k_decide", stream);
@@ -1129,13 +1220,13 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
   {
     FVH_TIMED("k_frame_finish", stream);
     hipLaunchKernelGGL(k_frame_finish, dim3(b.frame_count), dim3(256), 0, stream, b.frames, b.states,
-                       b.tiles, b.tile_len, b.tile_off, b.results, 1);
+                       b.tiles, b.tile_len, b.tile_off, b.results, 1, (char*)b.folded);
   }
   {
     FVH_TIMED("k_emit", stream);
     hipLaunchKernelGGL((k_emit<FT>), dim3((b.shard_total + 3) / 4), dim3(256), 0, stream, b.frames,
                        b.states, b.tiles, b.shards, b.shard_total, (const FT*)b.folded, b.recs,
-                       b.scratch, b.tile_off);
+                       b.scratch, b.tile_off, (char*)b.folded);
   }
 }
 
@@ -1149,10 +1240,14 @@ void launch_encode(const EncodeBuffers& b, hipStream_t stream) {
 template <typename FT>
 void launch_reconstruct(const DecodeBuffers& b, hipStream_t stream) {
   FT* folded = (FT*)b.folded;
-  {
+  if (b.any_band) {
+    launch_band<false, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.any_med,
+                           b.any_grad, stream);
+  }
+  if (b.any_nonband) {
     FVH_TIMED("k_wavefront_dec", stream);
     hipLaunchKernelGGL((k_wavefront<false, FT>), dim3((b.tile_total + 3) / 4), dim3(256), 0, stream,
-                       b.frames, b.states, b.tiles, b.tile_total, folded);
+                       b.frames, b.states, b.tiles, b.tile_total, folded, (char*)folded);
   }
   if (b.any_wide) {
     {

@@ -5,7 +5,8 @@
 
 namespace fvh {
 
-constexpr int kRingMax = 4096;   // widest tile handled by the wave-per-tile wavefront
+constexpr int kRingMax = 4096;     // widest tile handled by the wave-per-tile wavefront
+constexpr int kBandRingMax = 512;  // widest tile handled by the band wavefront
 
 struct EncodeBuffers {
   FrameDesc* frames;
@@ -28,6 +29,10 @@ struct EncodeBuffers {
   int any_lossless;
   int any_lossy;
   int any_wide;
+  int any_band;          // some frame uses the band wavefront
+  int any_nonband;       // some frame needs k_wavefront
+  int any_med;           // some frame may predict with MED
+  int any_grad;          // some frame may predict with the clamped gradient
 };
 
 struct DecodeBuffers {
@@ -36,6 +41,8 @@ struct DecodeBuffers {
   TileDesc* tiles;
   ShardDesc* shards;
   DShard* dshards;
+  int32_t* lists;        // [0, shard_total): order-0 shards, [shard_total, 2x): others
+  int32_t* list_count;   // [0] order-0, [1] others
   void* folded;
   Results* results;
   int frame_count;
@@ -44,6 +51,10 @@ struct DecodeBuffers {
   int wide_folded;
   int any_wide;
   int wavefront;
+  int any_band;
+  int any_nonband;
+  int any_med;
+  int any_grad;
 };
 
 void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,
@@ -54,5 +65,8 @@ void launch_encode(const EncodeBuffers& b, hipStream_t stream);
 void launch_decode(const DecodeBuffers& b, hipStream_t stream);
 template <typename FT>
 void launch_reconstruct(const DecodeBuffers& b, hipStream_t stream);
+template <bool kEncode, typename FT>
+void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* tiles, int tile_total,
+                 FT* folded, bool med, bool plain, hipStream_t stream);
 
 }  // namespace fvh

@@ -164,3 +164,41 @@ def test_rans_reciprocal_is_exact():
                             np.array([(1 << 31) - 1], np.uint64)])
         q = ((x * np.uint64(m)) >> np.uint64(32)) >> np.uint64(c - 1)
         assert np.array_equal(q, x // np.uint64(f)), f
+
+
+def _reference_quantize(r, d):
+    mag = (np.abs(r) + d // 2) // d
+    return np.where(r < 0, -mag, mag)
+
+
+@pytest.mark.parametrize("bit_depth", (8, 10, 12))
+def test_band_quantiser_signed_offset_is_exact(bit_depth):
+    """The band wavefront's q = floor((r + off) * m >> s) - K, off = h + K*d
+    (r >= 0) or d - 1 - h + K*d (r < 0), s = bd + 2 + ceil(log2 d), equals
+    sign(r) * floor((|r| + d/2) / d) for every residual and reachable step."""
+    maxv = (1 << bit_depth) - 1
+    r = np.arange(-maxv, maxv + 1, dtype=np.int64)
+    steps = {1 + ((100 - q) * (1 << (bit_depth - 8)) + den - 1) // den
+             for q in range(1, 101) for den in (6, 10, 20, 24)}
+    for d in sorted(x for x in steps if x > 1):
+        h = d // 2
+        K = -(-maxv // d)
+        c = (d - 1).bit_length()
+        s = bit_depth + 2 + c
+        m = -(-(1 << s) // d)
+        n = r + np.where(r < 0, d - 1 - h + K * d, h + K * d)
+        assert n.min() >= 0 and int(n.max()) * m < (1 << 32) and m < (1 << 24)
+        q = ((n * m) >> s) - K
+        assert np.array_equal(q, _reference_quantize(r, d)), d
+
+
+def test_band_quantiser_16bit_is_exact():
+    maxv = 65535
+    r = np.arange(-maxv, maxv + 1, dtype=np.int64)
+    for d in (44, 45, 161, 417, 556, 1000, 2561, 4225):
+        h, K = d // 2, -(-maxv // d)
+        m = -(-(1 << 32) // d)
+        n = r + np.where(r < 0, d - 1 - h + K * d, h + K * d)
+        assert n.max() < (1 << 18)
+        q = ((n.astype(np.uint64) * np.uint64(m)) >> np.uint64(32)).astype(np.int64) - K
+        assert np.array_equal(q, _reference_quantize(r, d)), d

@@ -116,6 +116,16 @@ CASES = [
     ("gray", 1, 1, 8, 100, (256, 128)),
     ("gray", 5, 3, 10, 20, (2, 2)),
     ("rgb444", 7, 300, 8, 90, (3, 64)),
+    # widths, pitches and tiles multiples of 8: the band wavefront path
+    ("rgb444", 256, 144, 8, 90, (64, 32)),
+    ("rgb444", 512, 136, 8, 66, (256, 128)),
+    ("gray", 320, 200, 10, 90, (128, 64)),      # gray10 MED pass on the band path
+    ("gray", 256, 136, 16, 95, (128, 128)),     # gray16 refinement on the band path
+    ("gray", 256, 136, 16, 60, (256, 120)),
+    ("yuv422", 384, 96, 10, 75, (64, 32)),
+    ("gray", 512, 70, 12, 70, (512, 70)),
+    ("rgb444", 96, 88, 10, 80, (32, 40)),       # tile narrower than one band period
+    ("gray", 64, 300, 8, 100, (64, 300)),       # lossless encode, band decode
 ]
 
 
