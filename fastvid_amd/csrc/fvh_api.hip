@@ -482,7 +482,7 @@ int fvh_decode_check_host(const uint8_t* bytes, int64_t size, const fvh_stream_i
 }
 
 struct DecodePlanLayout {
-  int64_t frames, states, tiles, shards, dshards, lists, folded, total;
+  int64_t frames, states, tiles, shards, dshards, lists, o0tabs, folded, total;
   int64_t tiles_n, shards_n, samples;
   bool wide_folded, any_wide;
 };
@@ -511,6 +511,7 @@ static DecodePlanLayout decode_layout(const fvh_stream_info* infos, int32_t coun
   L.shards = take(L.shards_n * (int64_t)sizeof(ShardDesc));
   L.dshards = take(L.shards_n * (int64_t)sizeof(DShard));
   L.lists = take((2 * L.shards_n + 4) * 4);
+  L.o0tabs = take(L.shards_n * (int64_t)sizeof(O0Tab));
   L.folded = take((L.samples + 8) * (L.wide_folded ? 4 : 2));
   L.total = off;
   return L;
@@ -592,6 +593,7 @@ int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
   b.dshards = (DShard*)(ws + L.dshards);
   b.lists = (int32_t*)(ws + L.lists);
   b.list_count = b.lists + 2 * L.shards_n;
+  b.o0tabs = (O0Tab*)(ws + L.o0tabs);
   b.folded = ws + L.folded;
   b.results = (Results*)results;
   b.frame_count = count;

@@ -138,6 +138,21 @@ struct DShard {
   int32_t count;
 };
 
+// Parsed order-0 table of one shard (k_o0_parse -> k_dec_o0), indexed by the
+// shard's position in the order-0 list.  ent[k] = cum_k | value_k << 13 for
+// the nsym ranks in table order and ent[nsym] = 4096 (the end of the slot
+// range), so a rank's frequency is the difference of two adjacent entries.
+struct O0Tab {
+  int32_t nsym;            // 0: the shard's table was rejected or not decoded
+  int32_t pad[3];
+  int32_t start[4];        // rANS lane byte offsets within the body
+  int32_t length[4];
+  uint32_t state[4];
+  uint32_t sink[4];        // per-lane target of the decoder's masked-off stores
+  uint32_t ent[256];
+};
+static_assert(sizeof(O0Tab) == 1104, "O0Tab layout");
+
 struct Results {           // per frame, read back by the host
   int64_t stream_length;
   int32_t status;

@@ -396,231 +396,356 @@ __global__ __launch_bounds__(64) void k_dec_other(const FrameDesc* __restrict__ 
   }
 }
 
-// Order-0 rANS shards (decode.cu:441-549): four shards per wave, 16 lanes per
-// shard.  Per shard: the table bytes are staged in LDS and parsed by one lane
-// with the reference's checks; the 16 lanes fill the 4096-slot rank LUT; four
-// lanes run the rANS lanes, each with an 8-byte lookahead of its byte stream
-// so the state recurrence only waits on two LDS lookups per symbol.
-constexpr int kO0Groups = 4;
-constexpr int kO0Stage = 1312;  // >= any valid table + lane lengths + states
+// Forward byte queue over a byte range of global memory, read as the aligned
+// 16-byte chunks that hold it (a chunk never crosses a page, so its bytes
+// outside the range are readable whenever the range is).  q0:q1 hold the next
+// `qn` bytes least significant first; `nxt` is the following chunk, requested
+// as soon as the current one is taken, so a byte read waits on memory only
+// when a whole chunk was consumed before its successor arrived.  All state is
+// shifted, never indexed, so it stays in registers.
+struct ByteQueue {
+  const uint4* cp;
+  uint64_t q0, q1;
+  uint4 nxt;
+  int qn;    // bytes in q0:q1
+  int left;  // bytes of the range not yet in q0:q1
+  int c;     // chunk index of q0:q1
+  int last;  // index of the last chunk holding bytes of the range
 
-struct O0Meta {
-  int valid;
-  int n;
-  int start[4];
-  int length[4];
-  uint32_t state[4];
+  __device__ __forceinline__ void init(const uint8_t* p, int length) {
+    const int head = (int)((uintptr_t)p & 15);
+    cp = (const uint4*)(p - head);
+    c = 0;
+    last = (head + length - 1) >> 4;
+    q0 = q1 = 0;
+    nxt = make_uint4(0, 0, 0, 0);
+    qn = 0;
+    left = length;
+    if (length > 0) {
+      const uint4 first = cp[0];
+      nxt = cp[min(1, last)];
+      q0 = ((uint64_t)first.y << 32) | first.x;
+      q1 = ((uint64_t)first.w << 32) | first.z;
+      if (head >= 8) {
+        q0 = q1 >> (8 * (head - 8));
+        q1 = 0;
+      } else if (head > 0) {
+        q0 = (q0 >> (8 * head)) | (q1 << (64 - 8 * head));
+        q1 >>= 8 * head;
+      }
+      qn = min(16 - head, length);
+      left = length - qn;
+    }
+  }
+
+  // Next byte of the range; false past its end.
+  __device__ __forceinline__ bool take(uint32_t* byte) {
+    if (qn == 0) {
+      if (left == 0) return false;
+      ++c;
+      q0 = ((uint64_t)nxt.y << 32) | nxt.x;
+      q1 = ((uint64_t)nxt.w << 32) | nxt.z;
+      qn = min(16, left);
+      left -= qn;
+      // the chunk after this one, or this one again at the end of the range;
+      // requested only after the old one was copied, so the load can target
+      // the same registers instead of a copy that would wait for it
+      __builtin_amdgcn_sched_barrier(0);
+      nxt = cp[min(c + 1, last)];
+    }
+    *byte = (uint32_t)q0 & 0xffu;
+    q0 = (q0 >> 8) | (q1 << 56);
+    q1 >>= 8;
+    --qn;
+    return true;
+  }
+
+  __device__ __forceinline__ bool drained() const { return qn == 0 && left == 0; }
 };
+
+// Order-0 table parse (decode.cu:441-509), one thread per order-0 shard: the
+// sparse table (table log, symbol count, value deltas and frequencies as
+// canonical LEB128), three lane lengths and four initial states, with the
+// reference's checks; a rejected table sets status 5.
+__global__ __launch_bounds__(256) void k_o0_parse(const FrameDesc* __restrict__ frames,
+                                                  FrameState* __restrict__ states,
+                                                  const DShard* __restrict__ dshards,
+                                                  const int32_t* __restrict__ lists,
+                                                  const int32_t* __restrict__ list_count,
+                                                  O0Tab* __restrict__ tabs, char* gbase) {
+  const int item = blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= list_count[0]) return;
+  const DShard d = dshards[lists[item]];
+  O0Tab* tab = tabs + item;
+  FrameState* st = &states[d.frame];
+  if (st->status != 0) {
+    tab->nsym = 0;
+    return;
+  }
+  const FrameDesc& f = frames[d.frame];
+  const uint32_t max_folded = (uint32_t)f.max_sample * 2u;
+  const int body_length = d.body_length;
+  ByteQueue q;
+  q.init(as_global(gbase, (const uint8_t*)f.stream + d.body_offset), body_length);
+  int cursor = 0;
+  auto varint = [&](uint32_t* value) -> bool {
+    uint32_t result = 0;
+    for (int used = 0; used < 5; ++used) {
+      uint32_t byte;
+      if (!q.take(&byte)) return false;
+      ++cursor;
+      if (used == 4 && byte > 0x0f) return false;
+      result |= (byte & 0x7f) << (7 * used);
+      if ((byte & 0x80) == 0) {
+        if (varint_len(result) != used + 1) return false;
+        *value = result;
+        return true;
+      }
+    }
+    return false;
+  };
+  auto u32le = [&](uint32_t* value) -> bool {
+    uint32_t v = 0;
+    for (int k = 0; k < 4; ++k) {
+      uint32_t byte;
+      if (!q.take(&byte)) return false;
+      v |= byte << (8 * k);
+    }
+    cursor += 4;
+    *value = v;
+    return true;
+  };
+  int valid = 1;
+  uint32_t log_byte = 0;
+  if (!q.take(&log_byte) || log_byte != kRansTableLog) valid = 0;
+  cursor = 1;
+  uint32_t nsym = 0;
+  if (valid && !varint(&nsym)) valid = 0;
+  if (nsym == 0 || nsym > 255) valid = 0;
+  uint32_t previous = 0, assigned = 0;
+  for (uint32_t k = 0; valid && k < nsym; ++k) {
+    uint32_t delta = 0;
+    if (!varint(&delta) || (k != 0 && delta == 0) || previous + delta > max_folded) {
+      valid = 0;
+      break;
+    }
+    const uint32_t value = previous + delta;
+    uint32_t fr = (1u << kRansTableLog) - assigned;
+    if (k + 1 != nsym && !varint(&fr)) {
+      valid = 0;
+      break;
+    }
+    // a u32 wrap of assigned + fr would make the reference fill its table
+    // out of bounds; no defined output exists, the stream is rejected
+    if ((uint64_t)assigned + fr > (1u << kRansTableLog)) {
+      valid = 0;
+      break;
+    }
+    if (fr == 0 || assigned + fr > (1u << kRansTableLog) ||
+        (k + 1 != nsym && assigned + fr == (1u << kRansTableLog))) {
+      valid = 0;
+      break;
+    }
+    tab->ent[k] = assigned | (value << 13);
+    assigned += fr;
+    previous = value;
+  }
+  if (assigned != (1u << kRansTableLog)) valid = 0;
+  int32_t len[4] = {0, 0, 0, 0};
+  uint32_t total_payload = 0;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    uint32_t v = 0;
+    if (valid && !u32le(&v)) valid = 0;
+    len[l] = (int32_t)v;
+    total_payload += v;
+  }
+  uint32_t state[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    if (valid && !u32le(&state[l])) valid = 0;
+    if (valid && state[l] < kRansL) valid = 0;
+  }
+  if (valid) {
+    len[3] = (int32_t)((uint32_t)(body_length - cursor) - total_payload);
+    int64_t at = cursor;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      if (len[l] < 0) valid = 0;  // the reference fails such a lane later
+      tab->start[l] = (int32_t)at;
+      tab->length[l] = len[l];
+      tab->state[l] = state[l];
+      at += len[l];
+    }
+    if (at != body_length) valid = 0;
+  }
+  if (!valid) {
+    set_status(st, 5);
+    tab->nsym = 0;
+    return;
+  }
+  tab->ent[nsym] = 1u << kRansTableLog;
+  tab->nsym = (int32_t)nsym;
+}
+
+// Order-0 rANS lanes (decode.cu:511-549): sixteen shards per wave, one quad
+// of lanes per shard, lane r of the quad running rANS lane r.  The slot ->
+// rank map is a 256-entry bucket table in LDS: bucket b covers slots
+// [16 b, 16 b + 16) and holds the rank at its first slot (bits 0-7) and a mask
+// of the offsets 1..15 where a new rank starts (bits 17-31), so the rank of a
+// slot is the first rank plus a popcount.  Bucket and entry table take 2 KB
+// per shard.
+//
+// The symbol loop runs in rounds of four steps with uniform control flow: a
+// step renormalises with at most two bytes, so a round needs at most eight,
+// taken big-endian from a two-chunk window of the lane's bytes; the chunk
+// after the window is loaded one round ahead, unconditionally, and steps past
+// the lane's end store to a sink slot.  Every round thus issues the same
+// memory operations, and the only wait on memory is for that chunk.
+constexpr int kO0Shards = 16;
+
+// A 16-byte chunk as two little-endian quadwords.  (Chunks are kept as
+// scalars: a conditional between two vector-typed objects would be compiled
+// as a copy through private memory.)
+struct Chunk {
+  uint64_t lo, hi;
+};
+
+__device__ __forceinline__ Chunk load_chunk(const uint4* p) {
+  const uint4 v = *p;
+  return Chunk{((uint64_t)v.y << 32) | v.x, ((uint64_t)v.w << 32) | v.z};
+}
+
+// Eight bytes starting at byte `pos` (0..15) of the 32-byte window A:B, the
+// first byte most significant, as a funnel shift of two quadwords.
+__device__ __forceinline__ uint64_t window8_be(uint64_t a0, uint64_t a1, uint64_t b0, int pos) {
+  const bool low = pos < 8;
+  const uint64_t lo = low ? a0 : a1;
+  const uint64_t hi = low ? a1 : b0;
+  const uint32_t sh = 8u * (uint32_t)(pos & 7);
+  const uint64_t le = sh ? (lo >> sh) | (hi << (64u - sh)) : lo;
+  return __builtin_bswap64(le);
+}
 
 template <typename FT>
 __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ frames,
-                                               FrameState* __restrict__ states,
                                                const DShard* __restrict__ dshards,
                                                const int32_t* __restrict__ lists,
                                                const int32_t* __restrict__ list_count,
+                                               O0Tab* __restrict__ tabs,
+                                               FrameState* __restrict__ states,
                                                FT* __restrict__ folded, char* gbase) {
-  __shared__ __attribute__((aligned(16))) uint8_t lut_all[kO0Groups][4096];
-  __shared__ uint32_t ent_all[kO0Groups][256];
-  __shared__ uint16_t val_all[kO0Groups][256];
-  __shared__ O0Meta meta_all[kO0Groups];
+  __shared__ uint32_t bkt_all[kO0Shards][256];
+  __shared__ uint32_t ent_all[kO0Shards][256];
   const int lane = lane_id();
-  const int g = lane >> 4;
-  const int i = lane & 15;
-  const int total = list_count[0];
-  const int item = blockIdx.x * kO0Groups + g;
-  const bool has = item < total;
-  uint8_t* lut = lut_all[g];
+  const int g = lane >> 2;
+  const int r = lane & 3;
+  const int item = blockIdx.x * kO0Shards + g;
+  uint32_t* bkt = bkt_all[g];
   uint32_t* ent = ent_all[g];
-  uint16_t* val = val_all[g];
-  O0Meta& meta = meta_all[g];
-  DShard d = {};
-  FrameState* st = nullptr;
-  const FrameDesc* f = nullptr;
-  bool go = false;
-  if (has) {
-    d = dshards[lists[item]];
-    st = &states[d.frame];
-    f = &frames[d.frame];
-    go = st->status == 0;
-  }
-  const uint8_t* body =
-      as_global(gbase, go ? (const uint8_t*)f->stream + d.body_offset : (const uint8_t*)gbase);
-  const int64_t body_length = go ? d.body_length : 0;
-  // stage the table bytes (aliasing the LUT) as the aligned 16-byte chunks
-  // that hold them; such a chunk never crosses a page
-  const int head = (int)((uintptr_t)body & 15);
-  const int nstage = (int)min(body_length, (int64_t)kO0Stage);
-  const int nchunks = nstage > 0 ? (head + nstage + 15) >> 4 : 0;
-  const uint4* chunks = (const uint4*)(body - head);
-  for (int k = i; k < nchunks; k += 16) ((uint4*)lut)[k] = chunks[k];
-  __syncthreads();
-  if (go && i == 0) {
-    const uint32_t max_folded = (uint32_t)f->max_sample * 2u;
-    const uint8_t* b = lut + head;
-    // reads past the stage only happen for tables no valid stream can have
-    const int64_t limit = min(body_length, (int64_t)kO0Stage);
-    int valid = 1;
-    int64_t cursor = 0;
-    if (body_length < 1 || b[cursor++] != kRansTableLog) valid = 0;
-    uint32_t nsym = 0;
-    if (valid && !read_varint(b, limit, &cursor, &nsym)) valid = 0;
-    if (nsym == 0 || nsym > 255) valid = 0;
-    uint32_t previous = 0, assigned = 0;
-    for (uint32_t k = 0; valid && k < nsym; ++k) {
-      uint32_t delta = 0;
-      if (!read_varint(b, limit, &cursor, &delta) || (k != 0 && delta == 0) ||
-          previous + delta > max_folded) {
-        valid = 0;
-        break;
-      }
-      const uint32_t value = previous + delta;
-      uint32_t fr = (1u << kRansTableLog) - assigned;
-      if (k + 1 != nsym && !read_varint(b, limit, &cursor, &fr)) {
-        valid = 0;
-        break;
-      }
-      // a u32 wrap of assigned + fr would make the reference fill its table
-      // out of bounds; no defined output exists, the stream is rejected
-      if ((uint64_t)assigned + fr > (1u << kRansTableLog)) {
-        valid = 0;
-        break;
-      }
-      if (fr == 0 || assigned + fr > (1u << kRansTableLog) ||
-          (k + 1 != nsym && assigned + fr == (1u << kRansTableLog))) {
-        valid = 0;
-        break;
-      }
-      val[k] = (uint16_t)value;
-      ent[k] = (fr & 0xffffu) | ((uint32_t)assigned << 16);
-      assigned += fr;
-      previous = value;
-    }
-    if (assigned != (1u << kRansTableLog)) valid = 0;
-    int32_t total_payload = 0;
-    int* len = meta.length;  // LDS: indexed by a loop counter
-    for (int l = 0; l < 4; ++l) len[l] = 0;
-    for (int l = 0; valid && l < 3; ++l) {
-      if (cursor + 4 > limit) {
-        valid = 0;
-      } else {
-        len[l] = (int32_t)rd_u32(b + cursor);
-        total_payload = (int32_t)((uint32_t)total_payload + (uint32_t)len[l]);
-        cursor += 4;
-      }
-    }
-    if (valid && cursor + 16 > limit) valid = 0;
-    for (int l = 0; valid && l < 4; ++l) {
-      meta.state[l] = rd_u32(b + cursor + l * 4);
-      if (meta.state[l] < kRansL) valid = 0;
-    }
-    if (valid) {
-      cursor += 16;
-      len[3] = (int32_t)((uint32_t)(int32_t)(body_length - cursor) - (uint32_t)total_payload);
-      if (len[3] < 0) valid = 0;
-      for (int l = 0; l < 3; ++l) {
-        if (len[l] < 0) valid = 0;  // the reference fails such a lane later
-      }
-      int64_t c = cursor;
-      for (int l = 0; l < 4; ++l) {
-        meta.start[l] = (int)c;
-        c += len[l];
-      }
-      if (c != body_length) valid = 0;
-    }
-    meta.valid = valid;
-    meta.n = (int)nsym;
-    if (!valid) set_status(st, 5);
+  O0Tab* tab = tabs + item;
+  const int nsym = item < list_count[0] ? tab->nsym : 0;
+  if (nsym > 0) {
+    const uint4* src = (const uint4*)tab->ent;
+    for (int j = r; 4 * j <= nsym; j += 4) ((uint4*)ent)[j] = src[j];
   }
   __syncthreads();
-  const bool ok = go && meta.valid;
-  // rank LUT: lane i fills slots [256 i, 256 i + 256), four per store
-  if (ok) {
-    const int nsym = meta.n;
-    const int lo_slot = i * 256;
-    int rk = 0;
-    int next_cum = nsym > 1 ? (int)(ent[1] >> 16) : (1 << kRansTableLog);
-    while (next_cum <= lo_slot) {
-      ++rk;
-      next_cum = rk + 1 < nsym ? (int)(ent[rk + 1] >> 16) : (1 << kRansTableLog);
+  if (nsym > 0) {
+    // buckets [64 r, 64 r + 64): start from the rank holding slot 1024 r
+    auto cum = [&](int k) { return (int)(ent[k] & 0x1fffu); };
+    int lo = 0, hi = nsym;  // cum(lo) <= 1024 r < cum(hi)
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (cum(mid) <= 1024 * r) lo = mid;
+      else hi = mid;
     }
-    uint32_t* lut_w = (uint32_t*)lut + i * 64;
-    for (int w = 0; w < 64; ++w) {
-      uint32_t word = 0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int slot = lo_slot + w * 4 + q;
-        while (next_cum <= slot) {
-          ++rk;
-          next_cum = rk + 1 < nsym ? (int)(ent[rk + 1] >> 16) : (1 << kRansTableLog);
-        }
-        word |= (uint32_t)rk << (8 * q);
+    int k = lo;
+    int next = cum(k + 1);
+    for (int bi = 0; bi < 64; ++bi) {
+      const int base = (64 * r + bi) * 16;
+      while (next <= base) {
+        ++k;
+        next = cum(k + 1);
       }
-      lut_w[w] = word;
+      uint32_t mask = 0;
+      int j = k + 1;
+      int cj = next;
+      while (cj < base + 16) {
+        mask |= 1u << (cj - base);
+        ++j;
+        cj = cum(j);
+      }
+      bkt[64 * r + bi] = (uint32_t)k | (mask << 16);
     }
   }
   __syncthreads();
-  if (!ok || i >= 4) return;
-  // rANS lane i.  Its bytes are read as aligned 16-byte chunks: `cur` is the
-  // chunk being consumed (next byte at `pos`), `nxt` the following chunk,
-  // requested as soon as `cur` becomes current.  Only chunks holding at least
-  // one byte of the lane are read.  The window `bb` holds `avail` bytes, most
-  // significant first, and is topped up whenever it falls to four bytes, so
-  // the (at most two) renormalisation reads of a step never wait on memory.
-  const int r = i;
-  const uint8_t* src = body + meta.start[r];
-  const int length = meta.length[r];
-  uint32_t x = meta.state[r];
-  FT* out = folded + d.folded_offset;
+  if (nsym == 0) return;
+  const DShard d = dshards[lists[item]];
+  const FrameDesc& f = frames[d.frame];
+  const uint8_t* body = as_global(gbase, (const uint8_t*)f.stream + d.body_offset);
+  const int length = tab->length[r];
+  uint32_t x = tab->state[r];
+  // aligned chunks holding the lane's bytes (a lane without bytes reads the
+  // body's first chunk, and never uses it)
+  const uint8_t* first = length > 0 ? body + tab->start[r] : body;
+  const int head = (int)((uintptr_t)first & 15);
+  const uint4* cp = (const uint4*)(first - head);
+  const int last = length > 0 ? (head + length - 1) >> 4 : 0;
+  const Chunk A0 = load_chunk(cp);
+  const Chunk B0 = load_chunk(cp + min(1, last));
+  const Chunk C0 = load_chunk(cp + min(2, last));
+  uint64_t a0 = A0.lo, a1 = A0.hi, b0 = B0.lo, b1 = B0.hi, c0 = C0.lo, c1 = C0.hi;
+  int ci = 0;       // chunk index of A
+  int pos = head;   // next byte within A:B, < 16 at the start of a round
+  int rem = length; // lane bytes not yet consumed
+  FT* out = folded + d.folded_offset + r;
+  FT* sink = (FT*)&tab->sink[r];
   const int n = d.count;
-  int pos = (int)((uintptr_t)src & 15);
-  const uint4* cp = (const uint4*)(src - pos);
-  int left = length;  // lane bytes not yet moved into the window
-  int c = 0;
-  uint4 cur = make_uint4(0, 0, 0, 0);
-  uint4 nxt = cur;
-  if (left > 0) cur = cp[0];
-  if (left > 16 - pos) nxt = cp[1];
-  uint64_t bb = 0;
-  int avail = 0;
+  const int nsteps = n > r ? (n - r + 3) >> 2 : 0;
   int bad = 0;
-  for (int idx = r; idx < n; idx += 4) {
-    while (avail <= 4 && left > 0) {
-      // dword pos/4 of the chunk, by shifts
-      const uint64_t half = (pos & 8) ? (((uint64_t)cur.w << 32) | cur.z)
-                                      : (((uint64_t)cur.y << 32) | cur.x);
-      const uint32_t w = (uint32_t)(half >> ((pos & 4) << 3));
-      const int skip = pos & 3;
-      const int take = min(4 - skip, left);
-      // byte `pos` to the top, later bytes below it
-      const uint32_t msb_first = __builtin_bswap32(w) << (8 * skip);
-      bb |= ((uint64_t)msb_first << 32) >> (8 * avail);
-      avail += take;
-      left -= take;
-      pos += take;
-      if (pos == 16) {
-        ++c;
-        pos = 0;
-        cur = nxt;
-        if (left > 16) nxt = cp[c + 1];
-      }
+  // settle the window loads here: otherwise the wait for them stays at the
+  // top of the loop, where it would also wait for every earlier store
+  __builtin_amdgcn_s_waitcnt(0);
+  for (int s0 = 0; s0 < nsteps; s0 += 4) {
+    uint64_t bb = window8_be(a0, a1, b0, pos);
+    int used = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const bool live = s0 + t < nsteps;
+      const uint32_t slot = x & ((1u << kRansTableLog) - 1);
+      const uint32_t e = bkt[slot >> 4];
+      const int rank = (int)(e & 0xffu) + __popc((e >> 16) & ((2u << (slot & 15u)) - 2u));
+      const uint32_t lo_e = ent[rank];
+      const uint32_t hi_e = ent[rank + 1];
+      const uint32_t c = lo_e & 0x1fffu;
+      const uint32_t fr = (hi_e & 0x1fffu) - c;
+      FT* dst = live ? out + 4 * (s0 + t) : sink;
+      *dst = (FT)(lo_e >> 13);
+      const uint32_t xd = __umul24(fr, x >> kRansTableLog) + slot - c;
+      // one byte below 2^23, two below 2^15
+      const int nb = live ? (int)(xd < kRansL) + (int)(xd < (1u << 15)) : 0;
+      const uint32_t top = (uint32_t)(bb >> 48);
+      const uint32_t x1 = (xd << 8) | (top >> 8);
+      const uint32_t x2 = (xd << 16) | top;
+      x = !live ? x : nb == 0 ? xd : nb == 1 ? x1 : x2;
+      bb <<= 8 * nb;
+      used += nb;
     }
-    const uint32_t slot = x & ((1u << kRansTableLog) - 1);
-    const int sym = lut[slot];
-    const uint32_t e = ent[sym];
-    out[idx] = (FT)val[sym];
-    x = __umul24(e & 0xffffu, x >> kRansTableLog) + slot - (e >> 16);
-    while (x < kRansL) {
-      if (avail == 0) {
-        bad = 1;
-        break;
-      }
-      x = (x << 8) | (uint32_t)(bb >> 56);
-      bb <<= 8;
-      --avail;
-    }
-    if (bad) break;
+    bad |= used > min(rem, 8);
+    rem -= used;
+    pos += used;
+    const bool adv = pos >= 16;
+    a0 = adv ? b0 : a0;
+    a1 = adv ? b1 : a1;
+    b0 = adv ? c0 : b0;
+    b1 = adv ? c1 : b1;
+    pos -= adv ? 16 : 0;
+    ci += adv ? 1 : 0;
+    const Chunk Cn = load_chunk(cp + min(ci + 2, last));
+    c0 = Cn.lo;
+    c1 = Cn.hi;
   }
-  if (bad || left != 0 || avail != 0 || x != kRansL) set_status(st, 5);
+  if (bad || rem != 0 || x != kRansL) set_status(&states[d.frame], 5);
 }
 
 // predictor="serial": one thread per tile in raster order (decode.cu:608-656).
@@ -681,10 +806,16 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
                        b.shard_total, (char*)b.folded);
   }
   {
+    FVH_TIMED("k_o0_parse", stream);
+    hipLaunchKernelGGL(k_o0_parse, dim3((b.shard_total + 255) / 256), dim3(256), 0, stream,
+                       b.frames, b.states, b.dshards, b.lists, b.list_count, b.o0tabs,
+                       (char*)b.folded);
+  }
+  {
     FVH_TIMED("k_dec_o0", stream);
-    hipLaunchKernelGGL((k_dec_o0<FT>), dim3((b.shard_total + kO0Groups - 1) / kO0Groups), dim3(64),
-                       0, stream, b.frames, b.states, b.dshards, b.lists, b.list_count,
-                       (FT*)b.folded, (char*)b.folded);
+    hipLaunchKernelGGL((k_dec_o0<FT>), dim3((b.shard_total + kO0Shards - 1) / kO0Shards),
+                       dim3(64), 0, stream, b.frames, b.dshards, b.lists, b.list_count, b.o0tabs,
+                       b.states, (FT*)b.folded, (char*)b.folded);
   }
   {
     FVH_TIMED("k_dec_other", stream);

@@ -43,6 +43,7 @@ struct DecodeBuffers {
   DShard* dshards;
   int32_t* lists;        // [0, shard_total): order-0 shards, [shard_total, 2x): others
   int32_t* list_count;   // [0] order-0, [1] others
+  O0Tab* o0tabs;         // parsed order-0 tables, by order-0 list position
   void* folded;
   Results* results;
   int frame_count;
