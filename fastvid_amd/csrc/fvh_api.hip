@@ -119,7 +119,8 @@ Geometry fill_geometry(FrameDesc& f, int64_t width, int64_t height, int64_t tw, 
 }
 
 struct WorkspaceLayout {
-  int64_t frames, states, tiles, shards, recs, tile_len, tile_off, folded, scratch, recon, dshards;
+  int64_t frames, states, tiles, shards, recs, tile_len, tile_off, folded, scratch, recon, dshards,
+      sink;
   int64_t total;
 };
 
@@ -282,6 +283,7 @@ int build_encode_plan(const fvh_encode_frame* in, int32_t count, EncodePlan& pla
   w.scratch = take(plan.shards * (int64_t)kScratchBytes);
   w.recon = plan.any_wide ? take(plan.samples * 2) : -1;
   w.dshards = -1;
+  w.sink = take(kSinkBytes);
   w.total = off;
   return FVH_OK;
 }
@@ -332,6 +334,7 @@ int fvh_encode_launch(const fvh_encode_frame* frames, int32_t frame_count, void*
   b.folded = ws + plan.ws.folded;
   b.scratch = ws + plan.ws.scratch;
   b.recon = plan.ws.recon >= 0 ? (uint16_t*)(ws + plan.ws.recon) : nullptr;
+  b.sink = ws + plan.ws.sink;
   b.results = (Results*)results;
   b.frame_count = frame_count;
   b.tile_total = (int)plan.tiles;
@@ -482,7 +485,7 @@ int fvh_decode_check_host(const uint8_t* bytes, int64_t size, const fvh_stream_i
 }
 
 struct DecodePlanLayout {
-  int64_t frames, states, tiles, shards, dshards, lists, o0tabs, folded, total;
+  int64_t frames, states, tiles, shards, dshards, lists, o0tabs, folded, sink, total;
   int64_t tiles_n, shards_n, samples;
   bool wide_folded, any_wide;
 };
@@ -513,6 +516,7 @@ static DecodePlanLayout decode_layout(const fvh_stream_info* infos, int32_t coun
   L.lists = take((2 * L.shards_n + 4) * 4);
   L.o0tabs = take(L.shards_n * (int64_t)sizeof(O0Tab));
   L.folded = take((L.samples + 8) * (L.wide_folded ? 4 : 2));
+  L.sink = take(kSinkBytes);
   L.total = off;
   return L;
 }
@@ -594,6 +598,7 @@ int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
   b.lists = (int32_t*)(ws + L.lists);
   b.list_count = b.lists + 2 * L.shards_n;
   b.o0tabs = (O0Tab*)(ws + L.o0tabs);
+  b.sink = ws + L.sink;
   b.folded = ws + L.folded;
   b.results = (Results*)results;
   b.frame_count = count;

@@ -37,29 +37,42 @@ __device__ __forceinline__ int dpp_row_shr1(int v) {
   return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);
 }
 
+// Native 4 x u32 vector: plain loads and stores (HIP's uint4 wrapper is
+// copied with memcpy, which in the 32-bit case kept chunks in private memory).
+using v4u = uint32_t __attribute__((ext_vector_type(4)));
+
 template <typename FT>
 struct Vec8;  // eight residuals / samples of one lane
 
 template <>
 struct Vec8<uint16_t> {
-  uint4 v;
+  v4u v;
+  static constexpr int kQuads = 1;  // 16-byte quads per chunk
   __device__ __forceinline__ uint32_t get(int j) const {
     const uint32_t w = j < 2 ? v.x : j < 4 ? v.y : j < 6 ? v.z : v.w;
     return (j & 1) ? (w >> 16) : (w & 0xffffu);
   }
-  __device__ __forceinline__ void load(const uint16_t* p) { v = *(const uint4*)p; }
-  __device__ __forceinline__ void store(uint16_t* p) const { *(uint4*)p = v; }
-  __device__ __forceinline__ void zero() { v = make_uint4(0, 0, 0, 0); }
+  __device__ __forceinline__ void load(const uint16_t* p) { v = *(const v4u*)p; }
+  __device__ __forceinline__ void store(uint16_t* p) const { *(v4u*)p = v; }
+  __device__ __forceinline__ void get_q(const v4u* q) { v = q[0]; }
+  __device__ __forceinline__ void put_q(v4u* q) const { q[0] = v; }
+  __device__ __forceinline__ void zero() { v = v4u{0, 0, 0, 0}; }
   __device__ __forceinline__ void set(int j, uint32_t x) {
-    uint32_t& w = j < 2 ? v.x : j < 4 ? v.y : j < 6 ? v.z : v.w;
+    const int k = j >> 1;
+    uint32_t w = k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
     if (j & 1) w = (w & 0xffffu) | (x << 16);
     else w = x & 0xffffu;
+    if (k == 0) v.x = w;
+    else if (k == 1) v.y = w;
+    else if (k == 2) v.z = w;
+    else v.w = w;
   }
 };
 
 template <>
 struct Vec8<uint32_t> {
-  uint4 a, b;
+  v4u a, b;
+  static constexpr int kQuads = 2;
   __device__ __forceinline__ uint32_t get(int j) const {
     switch (j) {
       case 0: return a.x; case 1: return a.y; case 2: return a.z; case 3: return a.w;
@@ -67,15 +80,23 @@ struct Vec8<uint32_t> {
     }
   }
   __device__ __forceinline__ void load(const uint32_t* p) {
-    a = *(const uint4*)p;
-    b = *(const uint4*)(p + 4);
+    a = *(const v4u*)p;
+    b = *(const v4u*)(p + 4);
   }
   __device__ __forceinline__ void store(uint32_t* p) const {
-    *(uint4*)p = a;
-    *(uint4*)(p + 4) = b;
+    *(v4u*)p = a;
+    *(v4u*)(p + 4) = b;
+  }
+  __device__ __forceinline__ void get_q(const v4u* q) {
+    a = q[0];
+    b = q[1];
+  }
+  __device__ __forceinline__ void put_q(v4u* q) const {
+    q[0] = a;
+    q[1] = b;
   }
   __device__ __forceinline__ void zero() {
-    a = make_uint4(0, 0, 0, 0);
+    a = v4u{0, 0, 0, 0};
     b = a;
   }
   __device__ __forceinline__ void set(int j, uint32_t x) {
@@ -140,20 +161,41 @@ __device__ __forceinline__ int med3_asm(int a, int b, int c) {
 
 // kEncode: IN = uint16 source samples (row pitch), OUT = residuals FT (tile rows)
 // decode:  IN = residuals FT (tile rows),          OUT = uint16 samples (row pitch)
+//
+// Memory traffic goes through per-row line buffers in LDS.  A "line" is 64
+// samples of a tile row (8 chunks of 8 samples, one chunk per group), and a
+// lane crosses into a new line every 8 groups; because lane i runs i groups
+// behind lane 0, the rows that start (or finish) a line in a given group are
+// exactly two, i and i + 8 for i = group mod 8 (start) or (group + 1) mod 8
+// (finish).  So in every group the 16 lanes of a tile load one line for each
+// of the two starting rows (lane l: row i + 8 (l / 8), chunk l mod 8) and
+// store the two finished lines, and each lane exchanges only its own chunk
+// with LDS.  Each memory instruction then covers whole 64-sample row
+// segments instead of 16 separate rows.  Loads are issued one line period (8
+// groups) ahead into eight register slots, one per phase of the period, and
+// written to the line buffer in the group the line starts.  The band period
+// P is rounded up to whole lines so line starts stay on that schedule.
 template <bool kEncode, bool kMed, typename FT>
-__global__ __launch_bounds__(256) void k_band(const FrameDesc* __restrict__ frames,
-                                              FrameState* __restrict__ states,
-                                              const TileDesc* __restrict__ tiles, int tile_total,
-                                              FT* __restrict__ folded, char* gbase) {
+__global__ __launch_bounds__(64) void k_band(const FrameDesc* __restrict__ frames,
+                                             FrameState* __restrict__ states,
+                                             const TileDesc* __restrict__ tiles, int tile_total,
+                                             FT* __restrict__ folded, char* gbase,
+                                             uint8_t* __restrict__ sink) {
   using InT = typename std::conditional<kEncode, uint16_t, FT>::type;
   using OutT = typename std::conditional<kEncode, FT, uint16_t>::type;
-  __shared__ uint4 ring_all[16][kBandRingMax / 8];
+  constexpr int kQi = Vec8<InT>::kQuads;
+  constexpr int kQo = Vec8<OutT>::kQuads;
+  constexpr int kRowChunks = 9;  // 8 chunks + 1 of padding against bank conflicts
+  __shared__ v4u lin_all[4][kBandRows][kRowChunks * kQi];
+  __shared__ v4u lout_all[4][kBandRows][kRowChunks * kQo];
+  __shared__ uint4 ring_all[4][kBandRingMax / 8];
   const int lane = lane_id();
   const int grp = lane >> 4;
   const int i = lane & 15;
-  const int wave = threadIdx.x >> 6;
-  const int t_idx = (blockIdx.x * 4 + wave) * 4 + grp;
-  uint4* ring = ring_all[wave * 4 + grp];
+  const int t_idx = blockIdx.x * 4 + grp;
+  uint4* ring = ring_all[grp];
+  v4u (*lin)[kRowChunks * kQi] = lin_all[grp];
+  v4u (*lout)[kRowChunks * kQo] = lout_all[grp];
 
   // ---- per row-group tile setup
   bool has = t_idx < tile_total;
@@ -178,14 +220,23 @@ __global__ __launch_bounds__(256) void k_band(const FrameDesc* __restrict__ fram
   const int W = has ? td.width : 0;
   const int H = has ? td.height : 0;
   const int nb = (H + kBandRows - 1) / kBandRows;
-  const int P = max((W + 7) & ~7, 128);
-  const int my_steps = has ? (nb - 1) * P + W + kBandSkew : 0;
-  const int total = (int)wave_max((uint32_t)my_steps);
+  const int GW = ((W + 63) & ~63) >> 3;          // column groups of a row, whole lines
+  // band period in groups: whole lines, and at least the 16 groups after
+  // which lane 15 has written the column lane 0 of the next band reads
+  const int GP = max(GW, kBandRows);
+  const int WG = W >> 3;                          // column groups holding samples
+  const int my_groups = has ? (nb - 1) * GP + GW + kBandRows : 0;
+  const int total = (int)wave_max((uint32_t)my_groups);
   if (total == 0) return;
+  const uint32_t gp_magic = quant_magic(GP);      // floor(t / GP) = mulhi(t, magic)
 
   const QuantMagic qm = quant_params<FT>(step, bit_depth, max_sample);
-  const InT* in_base = nullptr;
-  OutT* out_base = nullptr;
+  // Loads without a line to fetch read the start of a valid buffer, stores
+  // without one go to the lane's sink slot: every group issues the same
+  // memory operations on every path.
+  const InT* in_base = (const InT*)folded;
+  OutT* out_base = (OutT*)sink;
+  OutT* const my_sink = (OutT*)(sink + lane * kBandSinkBytes);
   int64_t in_pitch = 0, out_pitch = 0;
   if (has) {
     const FrameDesc& f = frames[td.frame];
@@ -201,45 +252,61 @@ __global__ __launch_bounds__(256) void k_band(const FrameDesc* __restrict__ fram
       out_pitch = f.plane_width[td.plane];
     }
   }
-  auto row_ok = [&](int band) { return band < nb && band * kBandRows + i < H; };
+  // row and first column group of the line that tile row r has reached at
+  // local time t (groups since the row group started; lines start at t = 0
+  // mod 8): returns false where there is no such line
+  auto line_at = [&](int r, int t, int* row, int* cg) __attribute__((always_inline)) {
+    if (t < 0) return false;
+    const int bb = (int)__umulhi((uint32_t)t, gp_magic);
+    *cg = t - bb * GP;
+    *row = bb * kBandRows + r;
+    return bb < nb && *row < H;
+  };
+  // cooperative role of this lane: chunk c of a line of row `base + 8 * half`
+  const int c = i & 7;
+  const int half = i >> 3;
+  auto issue_load = [&](Vec8<InT>& slot, int G, int k) __attribute__((always_inline)) {
+    // line of row k + 8 half starting at group G + 8
+    const int r = k + 8 * half;
+    int row = 0, cg = 0;
+    const bool ok = line_at(r, G + 8 - r, &row, &cg) && cg + c < WG;
+    slot.load(in_base + (ok ? (int64_t)row * in_pitch + 8 * (cg + c) : 0));
+  };
 
-  // Lane state.  x is this lane's column for the current 8-step group (lanes
-  // start staggered by 8 steps: x < 0 means not started).  Input groups are
-  // prefetched four groups ahead into a ring of four register buffers.
-  int x = -kBandLag * i;
-  int b = 0;
+  // own lane state: cg = column group (negative before the row group starts),
+  // bo = band
+  int cg = -i;
+  int bo = 0;
   int L = 0, UL = 0;
   int r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0, r5 = 0, r6 = 0, r7 = 0;
   int bad = 0;
-  Vec8<InT> buf0, buf1, buf2, buf3;
-  buf0.zero();
-  buf1.zero();
-  buf2.zero();
-  buf3.zero();
-  // position of the group `ahead` groups after (x, b)
-  auto load_ahead = [&](Vec8<InT>& dst, int ahead) {
-    int nx = x + ahead * kBandLag, nbd = b;
-    if (nx >= P) {
-      nx -= P;
-      ++nbd;
-    }
-    if (nx >= 0 && nx < W && row_ok(nbd)) {
-      dst.load(in_base + (int64_t)(nbd * kBandRows + i) * in_pitch + nx);
-    }
-  };
-  load_ahead(buf0, 0);
-  load_ahead(buf1, 1);
-  load_ahead(buf2, 2);
-  load_ahead(buf3, 3);
   uint4 rv = make_uint4(0, 0, 0, 0);
-  if (i == 0 && x >= 0 && x < W && b > 0 && row_ok(b)) rv = ring[x >> 3];
+  auto row_ok = [&](int band) __attribute__((always_inline)) {
+    return band < nb && band * kBandRows + i < H;
+  };
 
-  auto group = [&](Vec8<InT>& cur) {
-    const bool act = x >= 0 && x < W && row_ok(b);
-    if (x == 0) {
+  Vec8<InT> s0, s1, s2, s3, s4, s5, s6, s7;
+  issue_load(s0, -8, 0);
+  issue_load(s1, -7, 1);
+  issue_load(s2, -6, 2);
+  issue_load(s3, -5, 3);
+  issue_load(s4, -4, 4);
+  issue_load(s5, -3, 5);
+  issue_load(s6, -2, 6);
+  issue_load(s7, -1, 7);
+
+  auto group = [&](Vec8<InT>& slot, int G, int k) __attribute__((always_inline)) {
+    // (1) the line starting now for rows k and k + 8 goes to the line buffer
+    slot.put_q(&lin[k + 8 * half][c * kQi]);
+    // (2) this lane's chunk
+    const bool act = cg >= 0 && cg < WG && row_ok(bo);
+    const int x = cg * kBandLag;
+    if (cg == 0) {
       L = 0;
       UL = 0;
     }
+    Vec8<InT> cur;
+    cur.get_q(&lin[i][(cg & 7) * kQi]);
     Vec8<OutT> out;
     out.zero();
     int rr[8] = {r0, r1, r2, r3, r4, r5, r6, r7};
@@ -270,35 +337,50 @@ __global__ __launch_bounds__(256) void k_band(const FrameDesc* __restrict__ fram
     }
     r0 = rr[0]; r1 = rr[1]; r2 = rr[2]; r3 = rr[3];
     r4 = rr[4]; r5 = rr[5]; r6 = rr[6]; r7 = rr[7];
-    if (act) {
-      out.store(out_base + (int64_t)(b * kBandRows + i) * out_pitch + x);
-      if (i == kBandRows - 1 && b + 1 < nb) {
-        ring[x >> 3] = make_uint4((uint32_t)r0 | ((uint32_t)r1 << 16),
-                                  (uint32_t)r2 | ((uint32_t)r3 << 16),
-                                  (uint32_t)r4 | ((uint32_t)r5 << 16),
-                                  (uint32_t)r6 | ((uint32_t)r7 << 16));
-      }
+    out.put_q(&lout[i][(cg & 7) * kQo]);
+    if (act && i == kBandRows - 1 && bo + 1 < nb) {
+      ring[x >> 3] = make_uint4((uint32_t)r0 | ((uint32_t)r1 << 16),
+                                (uint32_t)r2 | ((uint32_t)r3 << 16),
+                                (uint32_t)r4 | ((uint32_t)r5 << 16),
+                                (uint32_t)r6 | ((uint32_t)r7 << 16));
     }
-    // refill this buffer with the group four ahead, then advance
-    load_ahead(cur, 4);
-    x += kBandLag;
-    if (x >= P) {
-      x -= P;
-      ++b;
+    // (3) store the lines finished now, rows k + 1 and k + 9 (mod 16)
+    {
+      const int r = ((k + 1) & 7) + 8 * half;
+      int row = 0, lcg = 0;
+      // lcg is the line's last column group here (7 mod 8)
+      const bool ok = line_at(r, G - r, &row, &lcg) && lcg - 7 + c < WG;
+      Vec8<OutT> done;
+      done.get_q(&lout[r][c * kQo]);
+      int ok_store = ok;
+      asm volatile("" : "+v"(ok_store));
+      done.store(ok_store ? out_base + (int64_t)row * out_pitch + 8 * (lcg - 7 + c) : my_sink);
+    }
+    if (kEncode) asm volatile("" : "+v"(bad));
+    // (4) the line starting at G + 8 for rows k and k + 8, into the slot just
+    // emptied (after its last use, so the load can target the slot's registers)
+    __builtin_amdgcn_sched_barrier(0);
+    issue_load(slot, G, k);
+    // advance
+    ++cg;
+    if (cg >= GP) {
+      cg = 0;
+      ++bo;
     }
     // ring words for the next group (written by lane 15 at least one group earlier)
     rv = make_uint4(0, 0, 0, 0);
-    if (i == 0 && x >= 0 && x < W && b > 0 && row_ok(b)) rv = ring[x >> 3];
+    if (i == 0 && cg >= 0 && cg < WG && bo > 0 && row_ok(bo)) rv = ring[cg];
   };
 
-  for (int t = 0; t < total; t += 4 * kBandLag) {
-    group(buf0);
-    if (t + kBandLag >= total) break;
-    group(buf1);
-    if (t + 2 * kBandLag >= total) break;
-    group(buf2);
-    if (t + 3 * kBandLag >= total) break;
-    group(buf3);
+  for (int G = 0; G < total; G += 8) {
+    group(s0, G + 0, 0);
+    group(s1, G + 1, 1);
+    group(s2, G + 2, 2);
+    group(s3, G + 3, 3);
+    group(s4, G + 4, 4);
+    group(s5, G + 5, 5);
+    group(s6, G + 6, 6);
+    group(s7, G + 7, 7);
   }
   if (kEncode) {
     const uint64_t any = ballot(bad != 0);
@@ -310,27 +392,27 @@ __global__ __launch_bounds__(256) void k_band(const FrameDesc* __restrict__ fram
 
 template <bool kEncode, typename FT>
 void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* tiles, int tile_total,
-                 FT* folded, bool med, bool plain, hipStream_t stream) {
-  const dim3 grid((tile_total + 15) / 16);
+                 FT* folded, uint8_t* sink, bool med, bool plain, hipStream_t stream) {
+  const dim3 grid((tile_total + 3) / 4);
   if (med) {
     FVH_TIMED(kEncode ? "k_band_enc" : "k_band_dec", stream);
-    hipLaunchKernelGGL((k_band<kEncode, true, FT>), grid, dim3(256), 0, stream, frames, states,
-                       tiles, tile_total, folded, (char*)folded);
+    hipLaunchKernelGGL((k_band<kEncode, true, FT>), grid, dim3(64), 0, stream, frames, states,
+                       tiles, tile_total, folded, (char*)folded, sink);
   }
   if (plain) {
     FVH_TIMED(kEncode ? "k_band_enc_grad" : "k_band_dec_grad", stream);
-    hipLaunchKernelGGL((k_band<kEncode, false, FT>), grid, dim3(256), 0, stream, frames, states,
-                       tiles, tile_total, folded, (char*)folded);
+    hipLaunchKernelGGL((k_band<kEncode, false, FT>), grid, dim3(64), 0, stream, frames, states,
+                       tiles, tile_total, folded, (char*)folded, sink);
   }
 }
 
 template void launch_band<true, uint16_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                          uint16_t*, bool, bool, hipStream_t);
+                                          uint16_t*, uint8_t*, bool, bool, hipStream_t);
 template void launch_band<true, uint32_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                          uint32_t*, bool, bool, hipStream_t);
+                                          uint32_t*, uint8_t*, bool, bool, hipStream_t);
 template void launch_band<false, uint16_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                           uint16_t*, bool, bool, hipStream_t);
+                                           uint16_t*, uint8_t*, bool, bool, hipStream_t);
 template void launch_band<false, uint32_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                           uint32_t*, bool, bool, hipStream_t);
+                                           uint32_t*, uint8_t*, bool, bool, hipStream_t);
 
 }  // namespace fvh

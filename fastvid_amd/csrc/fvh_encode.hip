@@ -1126,7 +1126,7 @@ template <typename FT>
 static void encode_pass(const EncodeBuffers& b, hipStream_t stream) {
   FT* folded = (FT*)b.folded;
   if (b.any_lossy && b.any_band) {
-    launch_band<true, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.any_med,
+    launch_band<true, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.sink, b.any_med,
                           b.any_grad, stream);
   }
   if (b.any_lossy && b.any_nonband) {
@@ -1241,7 +1241,7 @@ template <typename FT>
 void launch_reconstruct(const DecodeBuffers& b, hipStream_t stream) {
   FT* folded = (FT*)b.folded;
   if (b.any_band) {
-    launch_band<false, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.any_med,
+    launch_band<false, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.sink, b.any_med,
                            b.any_grad, stream);
   }
   if (b.any_nonband) {

@@ -7,6 +7,8 @@ namespace fvh {
 
 constexpr int kRingMax = 4096;     // widest tile handled by the wave-per-tile wavefront
 constexpr int kBandRingMax = 512;  // widest tile handled by the band wavefront
+constexpr int kBandSinkBytes = 32; // per-lane sink slot of the band wavefront (8 x uint32)
+constexpr int kSinkBytes = 256 * kBandSinkBytes;  // workspace sink region
 
 struct EncodeBuffers {
   FrameDesc* frames;
@@ -19,6 +21,7 @@ struct EncodeBuffers {
   void* folded;          // uint16 or uint32 residuals, tile order
   uint8_t* scratch;      // kScratchBytes per shard
   uint16_t* recon;       // tile-order reconstruction for tiles wider than kRingMax (or null)
+  uint8_t* sink;         // kSinkBytes target of masked-off stores
   Results* results;
   int frame_count;
   int tile_total;
@@ -44,6 +47,7 @@ struct DecodeBuffers {
   int32_t* lists;        // [0, shard_total): order-0 shards, [shard_total, 2x): others
   int32_t* list_count;   // [0] order-0, [1] others
   O0Tab* o0tabs;         // parsed order-0 tables, by order-0 list position
+  uint8_t* sink;         // kSinkBytes target of masked-off stores
   void* folded;
   Results* results;
   int frame_count;
@@ -68,6 +72,6 @@ template <typename FT>
 void launch_reconstruct(const DecodeBuffers& b, hipStream_t stream);
 template <bool kEncode, typename FT>
 void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* tiles, int tile_total,
-                 FT* folded, bool med, bool plain, hipStream_t stream);
+                 FT* folded, uint8_t* sink, bool med, bool plain, hipStream_t stream);
 
 }  // namespace fvh
