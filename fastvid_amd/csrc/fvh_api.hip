@@ -73,14 +73,16 @@ Geometry fill_geometry(FrameDesc& f, int64_t width, int64_t height, int64_t tw, 
   f.rows = (int32_t)ceil_div(height, th);
   const int64_t rows = f.rows;
   const int64_t last_h = height - (rows - 1) * th;
-  int64_t tile_acc = 0, shard_acc = 0, sample_acc = 0, bound = kHeaderBytes;
+  int64_t tile_acc = 0, shard_acc = 0, sample_acc = 0, folded_acc = 0, bound = kHeaderBytes;
+  auto pad8 = [](int64_t v) { return (v + 7) & ~int64_t{7}; };
   for (int p = 0; p < 3; ++p) {
     f.plane_tile_begin[p] = (int32_t)tile_acc;
     if (p >= f.plane_count) {
       f.plane_width[p] = 0;
-      f.plane_sample_base[p] = sample_acc;
+      f.plane_sample_base[p] = folded_acc;
       f.plane_shard_base[p] = shard_acc;
       f.row_shards[p] = 0;
+      f.row_samples[p] = 0;
       f.ntw[p] = 1;
       f.cols[p] = 1;
       continue;
@@ -93,8 +95,14 @@ Geometry fill_geometry(FrameDesc& f, int64_t width, int64_t height, int64_t tw, 
     f.plane_width[p] = pw;
     f.ntw[p] = (int32_t)ntw;
     f.cols[p] = (int32_t)cols;
-    f.plane_sample_base[p] = sample_acc;
+    f.plane_sample_base[p] = folded_acc;
     f.plane_shard_base[p] = shard_acc;
+    // residuals: tile-raster order, each tile padded to a multiple of 8
+    // elements so every tile and shard starts 16-byte aligned
+    const int64_t full_row_res = (cols - 1) * pad8(ntw * th) + pad8(last_w * th);
+    const int64_t last_row_res = (cols - 1) * pad8(ntw * last_h) + pad8(last_w * last_h);
+    f.row_samples[p] = full_row_res;
+    folded_acc += (rows - 1) * full_row_res + last_row_res;
     const int64_t full_row = (cols - 1) * ceil_div(ntw * th, kShardSymbols) +
         ceil_div(last_w * th, kShardSymbols);
     const int64_t last_row = (cols - 1) * ceil_div(ntw * last_h, kShardSymbols) +
@@ -113,7 +121,7 @@ Geometry fill_geometry(FrameDesc& f, int64_t width, int64_t height, int64_t tw, 
   f.total_samples = sample_acc;
   g.tiles = tile_acc;
   g.shards = shard_acc;
-  g.samples = sample_acc;
+  g.samples = folded_acc;
   g.bound = bound;
   return g;
 }
@@ -495,7 +503,10 @@ static DecodePlanLayout decode_layout(const fvh_stream_info* infos, int32_t coun
   for (int32_t i = 0; i < count; ++i) {
     L.tiles_n += infos[i].tile_count;
     L.shards_n += infos[i].total_shards;
-    L.samples = ((L.samples + 7) & ~int64_t{7}) + infos[i].total_samples;
+    FrameDesc scratch_desc = {};
+    const Geometry g = fill_geometry(scratch_desc, infos[i].width, infos[i].height,
+                                     infos[i].tile_width, infos[i].tile_height, infos[i].layout);
+    L.samples = ((L.samples + 7) & ~int64_t{7}) + g.samples;
     L.wide_folded |= infos[i].bit_depth == 16;
     const int64_t tw = infos[i].tile_width;
     const int64_t ctw = infos[i].layout == 1 ? (tw + 1) / 2 : tw;

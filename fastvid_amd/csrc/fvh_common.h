@@ -53,9 +53,11 @@ struct FrameDesc {
   uint16_t* out_plane[3];      // decode output planes (device, dense rows)
   int64_t pitch[3];            // row pitch of plane[] in samples
   int64_t plane_width[3];      // samples per row of each plane
-  int64_t plane_sample_base[3];// first folded sample of each plane within the frame
+  int64_t plane_sample_base[3];// first residual of each plane within the frame's residuals
   int64_t plane_shard_base[3]; // first shard of each plane within the frame
   int64_t row_shards[3];       // shards in one full-height row of tiles
+  int64_t row_samples[3];      // residuals in one full-height row of tiles (every tile's
+                               // residuals start on a multiple of 8 elements)
   uint8_t* stream;             // encode: output stream; decode: input stream
   int64_t stream_capacity;     // encode: output capacity; decode: stream size
   int64_t folded_base;         // first folded element of the frame in the batch

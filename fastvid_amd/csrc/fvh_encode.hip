@@ -387,10 +387,45 @@ __device__ __forceinline__ int write_varint_bytes(uint8_t* out, uint32_t value) 
   return n;
 }
 
-// One wave64 per shard; lane l holds symbols l, 64+l, ..., so symbol i sits
-// in lane i%64 and its Rice lane i%4 equals l%4.  Everything the reference
-// computes with shared atomics, a tree reduction and one serial thread
-// (encode.cu:195-431) becomes ballots, popcounts and wave scans here.
+// Residuals of one lane: eight consecutive symbols per 16-byte quad (u16)
+// or two quads (u32), as a native vector so they stay in registers.
+using q4u = uint32_t __attribute__((ext_vector_type(4)));
+
+template <typename FT>
+struct Chunk8;
+
+template <>
+struct Chunk8<uint16_t> {
+  q4u v;
+  __device__ __forceinline__ void load(const uint16_t* p) { v = *(const q4u*)p; }
+  __device__ __forceinline__ uint32_t get(int b) const {
+    const uint32_t w = b < 2 ? v.x : b < 4 ? v.y : b < 6 ? v.z : v.w;
+    return (b & 1) ? (w >> 16) : (w & 0xffffu);
+  }
+};
+
+template <>
+struct Chunk8<uint32_t> {
+  q4u a, c;
+  __device__ __forceinline__ void load(const uint32_t* p) {
+    a = *(const q4u*)p;
+    c = *(const q4u*)(p + 4);
+  }
+  __device__ __forceinline__ uint32_t get(int b) const {
+    switch (b) {
+      case 0: return a.x; case 1: return a.y; case 2: return a.z; case 3: return a.w;
+      case 4: return c.x; case 5: return c.y; case 6: return c.z; default: return c.w;
+    }
+  }
+};
+
+// One wave64 per shard.  Lane l holds symbols 512 c + 8 l + b (chunk c < 8,
+// b < 8), loaded with eight 16-byte loads issued together (every tile's
+// residuals start 16-byte aligned and are padded to whole quads), so symbol i
+// has Rice lane i % 4 = b % 4 and a 128-symbol block is one 16-lane row of
+// one chunk.  Everything the reference computes with shared atomics, a tree
+// reduction and one serial thread (encode.cu:195-431) becomes per-lane
+// counts, ballots, popcounts and wave scans here.
 template <typename FT>
 __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ frames,
                                                  FrameState* __restrict__ states,
@@ -400,7 +435,7 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
                                                  ShardRec* __restrict__ recs,
                                                  uint8_t* __restrict__ scratch) {
   __shared__ uint32_t hist_all[4][512];
-  __shared__ __attribute__((aligned(16))) uint8_t ranks_all[4][kShardSymbols];
+  __shared__ uint64_t zw_all[4][64];
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   const int s = blockIdx.x * 4 + wave;
@@ -415,34 +450,54 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
   const FT* v = folded + td.folded_base + local0;
   uint32_t* hist = hist_all[wave];
   for (int i = lane; i < 512; i += 64) hist[i] = 0;
+
+  // chunk c of this lane starts at symbol 512 c + 8 lane; quads past the
+  // shard's last symbol are not read
+  Chunk8<FT> ch[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int base = 512 * c + 8 * lane;
+    ch[c].load(v + (base < count ? base : 0));
+  }
   __builtin_amdgcn_wave_barrier();
 
-  uint32_t val[64];
   uint32_t lmax = 0;
   int unsup = 0;
   int zeros = 0, starts = 0, over64 = 0, over8192 = 0;
-  uint64_t prevZ = 0, myZ = 0;
+  uint32_t zmask[8];  // zero bits of each chunk's eight symbols
+  uint32_t prev_last = 0;  // symbol before this lane's chunk is zero
 #pragma unroll
-  for (int j = 0; j < 64; ++j) {
-    const int i = (j << 6) + lane;
-    const bool valid = i < count;
-    const uint32_t x = valid ? (uint32_t)v[i] : 0u;
-    val[j] = x;
-    lmax = max(lmax, x);
-    const uint64_t Z = ballot(valid && x == 0);
-    zeros += popc64(Z);
-    starts += popc64(Z & ~((Z << 1) | (prevZ >> 63)));
-    prevZ = Z;
-    myZ = lane == j ? Z : myZ;
-    over64 += popc64(ballot(valid && x > 64));
-    over8192 += popc64(ballot(valid && x > 8192));
-    if (valid) {
-      if (x < (uint32_t)kRansAlphabet) atomicAdd(&hist[x], 1u);
-      else unsup = 1;
+  for (int c = 0; c < 8; ++c) {
+    uint32_t zm = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int i = 512 * c + 8 * lane + b;
+      const bool valid = i < count;
+      const uint32_t x = valid ? ch[c].get(b) : 0u;
+      lmax = max(lmax, x);
+      zm |= (uint32_t)(valid && x == 0) << b;
+      over64 += valid && x > 64;
+      over8192 += valid && x > 8192;
+      if (valid) {
+        if (x < (uint32_t)kRansAlphabet) atomicAdd(&hist[x], 1u);
+        else unsup = 1;
+      }
     }
+    zmask[c] = zm;
+    zeros += __popc(zm);
+    // the symbol before this lane's chunk: lane - 1's last, or for lane 0 the
+    // previous chunk's last symbol (lane 63)
+    const uint32_t up = shfl_up(zm >> 7, 1);
+    prev_last = lane == 0 ? prev_last : up;
+    starts += __popc(zm & ~((zm << 1) | (prev_last & 1u)));
+    prev_last = shfl(zm >> 7, 63);  // for lane 0 of the next chunk
   }
   __builtin_amdgcn_wave_barrier();
   const uint32_t maxv = wave_max(lmax);
+  zeros = wave_sum(zeros);
+  starts = wave_sum(starts);
+  over64 = wave_sum(over64);
+  over8192 = wave_sum(over8192);
   bool supported = ballot(unsup != 0) == 0;
 
   ShardRec rec = {};
@@ -558,26 +613,36 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
       rec.sparse_len = (uint16_t)(hdr + shfl(len_incl, 63));
       rec.distinct = (uint16_t)distinct;
       rec.supported = 1;
-      // rank of every symbol, staged in LDS and written with 16-byte stores
-      // (the rANS lanes read it instead of the residuals)
+      // rank of every symbol, eight per lane and chunk, written as 8-byte
+      // stores (the rANS lanes read it instead of the residuals)
       __builtin_amdgcn_wave_barrier();
-      uint8_t* stage = ranks_all[wave];
+      uint8_t* ranks = sbase + kScratchRanks;
 #pragma unroll
-      for (int j = 0; j < 64; ++j) {
-        const int i = (j << 6) + lane;
-        if (i < count) stage[i] = (uint8_t)hist[val[j]];
+      for (int c = 0; c < 8; ++c) {
+        const int base = 512 * c + 8 * lane;
+        uint32_t lo = 0, hi = 0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const uint32_t x = base + b < count ? ch[c].get(b) : 0u;
+          const uint32_t rk = base + b < count ? hist[x] & 0xffu : 0u;
+          if (b < 4) lo |= rk << (8 * b);
+          else hi |= rk << (8 * (b - 4));
+        }
+        if (base < count) *(uint2*)(ranks + base) = make_uint2(lo, hi);
       }
-      __builtin_amdgcn_wave_barrier();
-      uint4* dst = (uint4*)(sbase + kScratchRanks);
-      const uint4* srcv = (const uint4*)stage;
-      for (int w = lane; w < (count + 15) / 16; w += 64) dst[w] = srcv[w];
     }
   }
 
   // ---------------- best of zero-run / Rice / block-pack (encode.cu:226-307)
   if (!force || !supported) {
     // long zero runs (> 64 symbols cost a 2-byte token): runs whose start p
-    // has zeros at p..p+64, found from this lane's 64-bit zero word and the next
+    // has zeros at p..p+64, found from the 64-bit zero word of symbols
+    // [64 lane, 64 lane + 64) and the next word
+    uint8_t* zb = (uint8_t*)zw_all[wave];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) zb[64 * c + lane] = (uint8_t)zmask[c];
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t myZ = zw_all[wave][lane];
     uint64_t zp = shfl_up(myZ, 1);
     uint64_t zn = __shfl_down(myZ, 1, 64);
     if (lane == 0) zp = 0;
@@ -594,13 +659,22 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
     const int longs = wave_sum(popc64(st_bits & m));
     const int64_t zero_bytes = (int64_t)(count - zeros) + over64 + over8192 + starts + longs;
 
-    int64_t block_bytes = 0;
+    // block b = chunk b / 4, row b % 4 of the wave: OR over the row
+    int block_part = 0;
 #pragma unroll
-    for (int b = 0; b < 32; ++b) {
-      const uint32_t o = wave_or(val[2 * b] | val[2 * b + 1]);
-      const int n = clampi(count - b * kBlockSymbols, 0, kBlockSymbols);
-      if (n > 0) block_bytes += 1 + (n * bit_length(o) + 7) / 8;
+    for (int c = 0; c < 8; ++c) {
+      uint32_t o = 0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) o |= ch[c].get(b) & (512 * c + 8 * lane + b < count ? ~0u : 0u);
+      o |= shfl_xor(o, 1);
+      o |= shfl_xor(o, 2);
+      o |= shfl_xor(o, 4);
+      o |= shfl_xor(o, 8);
+      const int blk = 4 * c + (lane >> 4);
+      const int n = clampi(count - blk * kBlockSymbols, 0, kBlockSymbols);
+      if ((lane & 15) == 0 && n > 0) block_part += 1 + (n * bit_length(o) + 7) / 8;
     }
+    const int64_t block_bytes = wave_sum(block_part);
 
     const int lane_count = count < 4 ? count : 4;
     const int bl = bit_length(maxv);
@@ -608,20 +682,25 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
     int64_t best_bytes = INT64_MAX;
     uint32_t best_lane[4] = {0, 0, 0, 0};
     for (int k = 0; k <= kMaxRice; ++k) {
-      uint32_t sum = 0;
+      // class r = symbol index mod 4 = b mod 4
+      uint32_t cs[4] = {0, 0, 0, 0};
       if (k < bl) {
 #pragma unroll
-        for (int j = 0; j < 64; ++j) sum += val[j] >> k;
-        sum += shfl_xor(sum, 4);
-        sum += shfl_xor(sum, 8);
-        sum += shfl_xor(sum, 16);
-        sum += shfl_xor(sum, 32);
+        for (int c = 0; c < 8; ++c) {
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            const uint32_t x = 512 * c + 8 * lane + b < count ? ch[c].get(b) : 0u;
+            cs[b & 3] += x >> k;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cs[r] = wave_sum(cs[r]);
       }
       int64_t bytes = 0;
       uint32_t lb[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const uint32_t cls = shfl(sum, r);
+        const uint32_t cls = cs[r];
         const uint32_t n_r = r < lane_count ? (uint32_t)((count - r + 3) / 4) : 0u;
         const uint64_t bits = (uint64_t)cls + (uint64_t)n_r * (uint64_t)(1 + k);
         lb[r] = r < lane_count ? (uint32_t)((bits + 7) / 8) : 0u;

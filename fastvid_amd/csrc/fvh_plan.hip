@@ -36,7 +36,8 @@ __global__ __launch_bounds__(256) void k_plan(const FrameDesc* __restrict__ fram
   const int64_t w = min(ntw, pw - x);
   const int64_t h = min((int64_t)f.tile_h, f.height - y);
   TileDesc td;
-  td.folded_base = f.folded_base + f.plane_sample_base[plane] + y * pw + x * h;
+  td.folded_base = f.folded_base + f.plane_sample_base[plane] + (int64_t)r * f.row_samples[plane] +
+      (int64_t)c * ((ntw * h + 7) & ~int64_t{7});
   td.src_offset = y * f.pitch[plane] + x;
   td.dst_offset = y * pw + x;
   td.frame = lo;
