@@ -366,12 +366,15 @@ __device__ __forceinline__ ClampFn norm_fn(uint32_t prefix, int count, int remai
 // y = m, with c = ceil(log2 freq) and m = ceil(2^(31+c) / freq):
 // floor(x / freq) = mul_hi(x, m) >> (c-1) for every state x < 2^31 (freq >= 2;
 // freq == 1 divides trivially).
+// m is computed as a double quotient: 2^(31+c) / freq lies in [2^31, 2^32)
+// and, when not an integer, is at least 1/freq >= 2^-12 from one, while the
+// quotient's rounding error is at most 2^-22, so its ceiling is exact.
 __device__ __forceinline__ uint2 rans_entry(uint32_t freq, uint32_t cum) {
   uint32_t c = 0;
   uint32_t m = 0;
   if (freq >= 2) {
     c = 32 - __builtin_clz(freq - 1);
-    m = (uint32_t)(((1ull << (31 + c)) + freq - 1) / freq);
+    m = (uint32_t)ceil((double)(1ull << (31 + c)) / (double)freq);
   }
   return make_uint2(freq | (cum << 13) | (c << 25), m);
 }
@@ -398,6 +401,20 @@ template <>
 struct Chunk8<uint16_t> {
   q4u v;
   __device__ __forceinline__ void load(const uint16_t* p) { v = *(const q4u*)p; }
+  // zero the elements from n on (0 <= n <= 8)
+  __device__ __forceinline__ void keep(int n) {
+    const uint32_t m0 = n >= 2 ? ~0u : n == 1 ? 0xffffu : 0u;
+    const uint32_t m1 = n >= 4 ? ~0u : n == 3 ? 0xffffu : 0u;
+    const uint32_t m2 = n >= 6 ? ~0u : n == 5 ? 0xffffu : 0u;
+    const uint32_t m3 = n >= 8 ? ~0u : n == 7 ? 0xffffu : 0u;
+    v.x &= m0;
+    v.y &= m1;
+    v.z &= m2;
+    v.w &= m3;
+  }
+  // a fresh copy for the optimiser: values unpacked before this point are
+  // not reused after it, which keeps them from staying live across phases
+  __device__ __forceinline__ void launder() { asm volatile("" : "+v"(v)); }
   __device__ __forceinline__ uint32_t get(int b) const {
     const uint32_t w = b < 2 ? v.x : b < 4 ? v.y : b < 6 ? v.z : v.w;
     return (b & 1) ? (w >> 16) : (w & 0xffffu);
@@ -411,6 +428,17 @@ struct Chunk8<uint32_t> {
     a = *(const q4u*)p;
     c = *(const q4u*)(p + 4);
   }
+  __device__ __forceinline__ void keep(int n) {
+    a.x = n > 0 ? a.x : 0u;
+    a.y = n > 1 ? a.y : 0u;
+    a.z = n > 2 ? a.z : 0u;
+    a.w = n > 3 ? a.w : 0u;
+    c.x = n > 4 ? c.x : 0u;
+    c.y = n > 5 ? c.y : 0u;
+    c.z = n > 6 ? c.z : 0u;
+    c.w = n > 7 ? c.w : 0u;
+  }
+  __device__ __forceinline__ void launder() { asm volatile("" : "+v"(a), "+v"(c)); }
   __device__ __forceinline__ uint32_t get(int b) const {
     switch (b) {
       case 0: return a.x; case 1: return a.y; case 2: return a.z; case 3: return a.w;
@@ -444,13 +472,11 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
   const TileDesc td = tiles[sd.tile];
   FrameState* st = &states[td.frame];
   if (!st->active) return;
+  uint32_t* hist = hist_all[wave];
   const int force = frames[td.frame].force_order0;
   const int64_t local0 = (int64_t)sd.local * kShardSymbols;
   const int count = (int)min((int64_t)kShardSymbols, (int64_t)td.width * td.height - local0);
   const FT* v = folded + td.folded_base + local0;
-  uint32_t* hist = hist_all[wave];
-  for (int i = lane; i < 512; i += 64) hist[i] = 0;
-
   // chunk c of this lane starts at symbol 512 c + 8 lane; quads past the
   // shard's last symbol are not read
   Chunk8<FT> ch[8];
@@ -458,6 +484,14 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
   for (int c = 0; c < 8; ++c) {
     const int base = 512 * c + 8 * lane;
     ch[c].load(v + (base < count ? base : 0));
+  }
+  for (int i = lane; i < 512; i += 64) hist[i] = 0;
+  // symbols past the shard's end read as zeros; they are taken out of the
+  // zero masks and of the histogram's count of zeros below
+  const bool full = count == kShardSymbols;
+  if (!full) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) ch[c].keep(clampi(count - (512 * c + 8 * lane), 0, 8));
   }
   __builtin_amdgcn_wave_barrier();
 
@@ -471,18 +505,15 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
     uint32_t zm = 0;
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      const int i = 512 * c + 8 * lane + b;
-      const bool valid = i < count;
-      const uint32_t x = valid ? ch[c].get(b) : 0u;
+      const uint32_t x = ch[c].get(b);
       lmax = max(lmax, x);
-      zm |= (uint32_t)(valid && x == 0) << b;
-      over64 += valid && x > 64;
-      over8192 += valid && x > 8192;
-      if (valid) {
-        if (x < (uint32_t)kRansAlphabet) atomicAdd(&hist[x], 1u);
-        else unsup = 1;
-      }
+      zm |= (uint32_t)(x == 0) << b;
+      over64 += x > 64;
+      over8192 += x > 8192;
+      if (x < (uint32_t)kRansAlphabet) atomicAdd(&hist[x], 1u);
+      else unsup = 1;
     }
+    if (!full) zm &= (1u << clampi(count - (512 * c + 8 * lane), 0, 8)) - 1u;
     zmask[c] = zm;
     zeros += __popc(zm);
     // the symbol before this lane's chunk: lane - 1's last, or for lane 0 the
@@ -492,6 +523,9 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
     starts += __popc(zm & ~((zm << 1) | (prev_last & 1u)));
     prev_last = shfl(zm >> 7, 63);  // for lane 0 of the next chunk
   }
+  if (!full && lane == 0) atomicSub(&hist[0], (uint32_t)(kShardSymbols - count));
+#pragma unroll
+  for (int c = 0; c < 8; ++c) ch[c].launder();
   __builtin_amdgcn_wave_barrier();
   const uint32_t maxv = wave_max(lmax);
   zeros = wave_sum(zeros);
@@ -616,6 +650,8 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
       // rank of every symbol, eight per lane and chunk, written as 8-byte
       // stores (the rANS lanes read it instead of the residuals)
       __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int c = 0; c < 8; ++c) ch[c].launder();
       uint8_t* ranks = sbase + kScratchRanks;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
@@ -623,8 +659,7 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
         uint32_t lo = 0, hi = 0;
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-          const uint32_t x = base + b < count ? ch[c].get(b) : 0u;
-          const uint32_t rk = base + b < count ? hist[x] & 0xffu : 0u;
+          const uint32_t rk = hist[ch[c].get(b)] & 0xffu;
           if (b < 4) lo |= rk << (8 * b);
           else hi |= rk << (8 * (b - 4));
         }
@@ -635,6 +670,8 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
 
   // ---------------- best of zero-run / Rice / block-pack (encode.cu:226-307)
   if (!force || !supported) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) ch[c].launder();
     // long zero runs (> 64 symbols cost a 2-byte token): runs whose start p
     // has zeros at p..p+64, found from the 64-bit zero word of symbols
     // [64 lane, 64 lane + 64) and the next word
@@ -665,7 +702,7 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
     for (int c = 0; c < 8; ++c) {
       uint32_t o = 0;
 #pragma unroll
-      for (int b = 0; b < 8; ++b) o |= ch[c].get(b) & (512 * c + 8 * lane + b < count ? ~0u : 0u);
+      for (int b = 0; b < 8; ++b) o |= ch[c].get(b);
       o |= shfl_xor(o, 1);
       o |= shfl_xor(o, 2);
       o |= shfl_xor(o, 4);
@@ -681,6 +718,7 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
     int best_k = 0;
     int64_t best_bytes = INT64_MAX;
     uint32_t best_lane[4] = {0, 0, 0, 0};
+#pragma unroll 1
     for (int k = 0; k <= kMaxRice; ++k) {
       // class r = symbol index mod 4 = b mod 4
       uint32_t cs[4] = {0, 0, 0, 0};
@@ -688,10 +726,7 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
 #pragma unroll
-          for (int b = 0; b < 8; ++b) {
-            const uint32_t x = 512 * c + 8 * lane + b < count ? ch[c].get(b) : 0u;
-            cs[b & 3] += x >> k;
-          }
+          for (int b = 0; b < 8; ++b) cs[b & 3] += ch[c].get(b) >> k;
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) cs[r] = wave_sum(cs[r]);

@@ -62,6 +62,9 @@ struct DecodeBuffers {
   int any_grad;
 };
 
+// Compute units of the current device (cached per device).
+int device_cus();
+
 void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,
                  int tile_total, hipStream_t stream);
 void launch_upload_frames(const FrameDesc* host_frames, int frame_count, FrameDesc* device_frames,
