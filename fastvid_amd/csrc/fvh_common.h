@@ -34,7 +34,9 @@ constexpr int kMaxRecordBytes = 8739;
 // Per-shard order-0 scratch layout (bytes).
 constexpr int kScratchTable = 0;        // 256 x {freq|cum<<13|c<<25, reciprocal} by rank
 constexpr int kScratchSparse = 2048;    // sparse table bytes (<= 1024)
-constexpr int kScratchRanks = 3072;     // rank of every symbol, one byte each (4096)
+constexpr int kScratchRanks = 3072;     // ranks, 1024 bytes per rANS lane r: the rank of symbol
+                                        // 4k + r at byte 1024 r + 1023 - k (the order the
+                                        // encoder visits them)
 constexpr int kScratchLanes = 7168;     // 4 lanes x kLaneBytes, filled from the end
 constexpr int kLaneBytes = 1536;        // proven bound for 1024 symbols
 constexpr int kScratchBytes = kScratchLanes + 4 * kLaneBytes;  // 13312

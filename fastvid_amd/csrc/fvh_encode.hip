@@ -647,23 +647,24 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
       rec.sparse_len = (uint16_t)(hdr + shfl(len_incl, 63));
       rec.distinct = (uint16_t)distinct;
       rec.supported = 1;
-      // rank of every symbol, eight per lane and chunk, written as 8-byte
-      // stores (the rANS lanes read it instead of the residuals)
+      // rank stream, one region per rANS lane r (symbols 4k + r) holding the
+      // rank of symbol 4k + r at byte 1023 - k: the rANS lanes visit symbols
+      // from the last one back, so each reads its region forwards.  Symbols
+      // b = r and b = r + 4 of a chunk are k and k + 1 of lane r, one 16-bit
+      // store at byte 1022 - k.
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int c = 0; c < 8; ++c) ch[c].launder();
       uint8_t* ranks = sbase + kScratchRanks;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
-        const int base = 512 * c + 8 * lane;
-        uint32_t lo = 0, hi = 0;
+        const int k = 128 * c + 2 * lane;  // lane r index of symbol b = r
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-          const uint32_t rk = hist[ch[c].get(b)] & 0xffu;
-          if (b < 4) lo |= rk << (8 * b);
-          else hi |= rk << (8 * (b - 4));
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t rk0 = hist[ch[c].get(r)] & 0xffu;
+          const uint32_t rk1 = hist[ch[c].get(r + 4)] & 0xffu;
+          *(uint16_t*)(ranks + 1024 * r + 1022 - k) = (uint16_t)(rk1 | (rk0 << 8));
         }
-        if (base < count) *(uint2*)(ranks + base) = make_uint2(lo, hi);
       }
     }
   }
@@ -780,15 +781,27 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
 
 // Order-0 rANS encode (encode.cu:398-418): one thread per rANS lane, four
 // consecutive threads per shard, 16 shards per wave (one wave per block).
-// The shard's rank-indexed entries live in LDS; each lane walks its symbols
-// backwards reading one rank byte per symbol from the rank stream, loaded two
-// 8-symbol chunks ahead so the state recurrence never waits on memory.
-// Division by the symbol frequency is the reciprocal multiply of rans_entry.
-// Emitted bytes go backwards from the end of the lane's scratch area, packed
-// four per store, so the area ends with the lane's bytes in decode order.
-// The last step decides the shard's mode: order-0 when forced or strictly
-// smaller than the best of three (encode.cu:423-429).
+// The shard's rank-indexed entries live in LDS.  A lane reads its rank
+// region forwards in rounds of four symbols, taken from a two-chunk window
+// whose next chunk is loaded a round ahead; division by the symbol frequency
+// is the reciprocal multiply of rans_entry.  Renormalisation bytes collect in
+// a 128-bit register queue, and every round stores its oldest eight bytes
+// (or, with fewer pending, a sink slot) to the end of the lane's scratch
+// area, backwards, which thus ends with the lane's bytes in decode order.
+// Control flow is uniform across a round, so the loop waits on memory only
+// for the rank chunk.  The last step decides the shard's mode: order-0 when
+// forced or strictly smaller than the best of three (encode.cu:423-429).
 constexpr int kRansShards = 16;
+
+// Eight bytes starting at byte `pos` (0..15) of the 32-byte window a0:a1:b0:b1,
+// the first byte least significant.
+__device__ __forceinline__ uint64_t window8_le(uint64_t a0, uint64_t a1, uint64_t b0, int pos) {
+  const bool low = pos < 8;
+  const uint64_t lo = low ? a0 : a1;
+  const uint64_t hi = low ? a1 : b0;
+  const uint32_t sh = 8u * (uint32_t)(pos & 7);
+  return sh ? (lo >> sh) | (hi << (64u - sh)) : lo;
+}
 
 template <typename FT>
 __global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frames,
@@ -797,8 +810,10 @@ __global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frame
                                              const ShardDesc* __restrict__ shards,
                                              int shard_total, const FT* __restrict__ folded,
                                              ShardRec* __restrict__ recs,
-                                             uint8_t* __restrict__ scratch) {
+                                             uint8_t* __restrict__ scratch,
+                                             uint8_t* __restrict__ sink) {
   (void)folded;
+  using q2 = uint64_t __attribute__((ext_vector_type(2)));
   __shared__ uint2 ent_all[kRansShards][256];
   const int lane = lane_id();
   const int g = lane >> 2;
@@ -827,73 +842,75 @@ __global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frame
   int nbytes = 0;
   uint32_t x = kRansL;
   if (act) {
-    const uint32_t* rw = (const uint32_t*)(sbase + kScratchRanks);
+    const int nsym = count > r ? (count - r + 3) >> 2 : 0;  // symbols of this lane
     uint8_t* lane_end = sbase + kScratchLanes + (r + 1) * kLaneBytes;
-    uint32_t* lane_end_w = (uint32_t*)lane_end;
-    const int start = count - 1 - ((count - 1 - r) & 3);
-    const int nsym = start >= 0 ? (start >> 2) + 1 : 0;  // symbols of this lane
-    const uint32_t shift = 8u * (uint32_t)r;
-    uint32_t acc = 0;      // emitted bytes not yet stored, most recent in the low byte
-    int pending = 0;
-    int stored_words = 0;
-    // ranks of symbols start, start-4, ... : word (start-4k)>>2 = (start>>2)-k
-    const int w0 = start >> 2;
-    uint32_t a[8], b[8];
+    // ranks: aligned chunks ci, ci+1 in a, b; a block of 16 symbols takes
+    // the 16 bytes at offset `head` of a:b, while chunk ci+2 loads
+    const uint8_t* rp = sbase + kScratchRanks + 1024 * r + (1024 - nsym);
+    const int head = (int)((uintptr_t)rp & 15);
+    const q2* cp = (const q2*)(rp - head);
+    const int last = nsym > 0 ? (head + nsym - 1) >> 4 : 0;
+    const q2 ca = cp[0];
+    const q2 cb = cp[min(1, last)];
+    uint64_t a0 = ca.x, a1 = ca.y, b0 = cb.x, b1 = cb.y;
+    int ci = 0;
+    // byte queue: `pend` bytes, the most recent in the low byte of qlo
+    uint64_t qlo = 0, qhi = 0;
+    int pend = 0, blocks = 0;
+    uint64_t* my_sink = (uint64_t*)(sink + lane * 16);
+    __builtin_amdgcn_s_waitcnt(0);
+    for (int j0 = 0; j0 < nsym; j0 += 16) {
+      const uint64_t r_lo = window8_le(a0, a1, b0, head);
+      const uint64_t r_hi = window8_le(a1, b0, b1, head);
+      const q2 cn = cp[min(ci + 2, last)];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = u < nsym ? rw[w0 - u] : 0u;
+      for (int rd = 0; rd < 4; ++rd) {
+        const uint32_t rk4 = (uint32_t)((rd < 2 ? r_lo : r_hi) >> (32 * (rd & 1)));
+        uint2 e[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) b[u] = 8 + u < nsym ? rw[w0 - 8 - u] : 0u;
-    for (int k0 = 0; k0 < nsym; k0 += 8) {
-      uint32_t cur[8];
+        for (int t = 0; t < 4; ++t) e[t] = ent[(rk4 >> (8 * t)) & 0xffu];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        cur[u] = a[u];
-        a[u] = b[u];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int k = k0 + 16 + u;
-        b[u] = k < nsym ? rw[w0 - k] : 0u;
-      }
-      uint2 e[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) e[u] = ent[(cur[u] >> shift) & 0xffu];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (k0 + u < nsym) {
-          const uint32_t freq = e[u].x & 0x1fffu;
-          const uint32_t cum = (e[u].x >> 13) & 0xfffu;
-          const uint32_t c = e[u].x >> 25;
+        for (int t = 0; t < 4; ++t) {
+          const bool live = j0 + 4 * rd + t < nsym;
+          const uint32_t freq = e[t].x & 0x1fffu;
+          const uint32_t cum = (e[t].x >> 13) & 0xfffu;
+          const uint32_t c = e[t].x >> 25;
           const uint32_t lim = freq << 19;
-          if (x >= lim) {
-            acc = (acc << 8) | (x & 0xffu);
-            x >>= 8;
-            if (++pending == 4) {
-              if (stored_words < kLaneBytes / 4) lane_end_w[-1 - stored_words] = acc;
-              ++stored_words;
-              pending = 0;
-            }
-            if (x >= lim) {
-              acc = (acc << 8) | (x & 0xffu);
-              x >>= 8;
-              if (++pending == 4) {
-                if (stored_words < kLaneBytes / 4) lane_end_w[-1 - stored_words] = acc;
-                ++stored_words;
-                pending = 0;
-              }
-            }
-          }
-          const uint32_t q = c ? (__umulhi(x, e[u].y) >> (c - 1)) : x;
-          x += __umul24(q, 4096u - freq) + cum;
+          const int nb = live ? (int)(x >= lim) + (int)((x >> 8) >= lim) : 0;
+          // append the nb low bytes of x, first the lowest
+          const uint32_t sh = 8u * (uint32_t)nb;
+          const uint32_t add = nb == 2 ? (((x & 0xffu) << 8) | ((x >> 8) & 0xffu)) : (x & 0xffu);
+          qhi = sh ? (qhi << sh) | (qlo >> (64u - sh)) : qhi;
+          qlo = sh ? (qlo << sh) | add : qlo;
+          pend += nb;
+          const uint32_t xs = x >> sh;
+          const uint32_t q = c ? (__umulhi(xs, e[t].y) >> (c - 1)) : xs;
+          const uint32_t xn = xs + __umul24(q, 4096u - freq) + cum;
+          x = live ? xn : x;
         }
+        // the oldest eight pending bytes, if there are eight, in memory order
+        const bool full = pend >= 8;
+        const uint32_t ds = 8u * (uint32_t)(pend - 8);  // < 64 when full
+        const uint64_t word = full ? (ds ? (qlo >> ds) | (qhi << (64u - ds)) : qlo) : 0ull;
+        const bool fits = 8 * (blocks + 1) <= kLaneBytes;
+        int st_ok = full && fits;
+        asm volatile("" : "+v"(st_ok));
+        *(st_ok ? (uint64_t*)(lane_end - 8 * (blocks + 1)) : my_sink) = word;
+        blocks += full ? 1 : 0;
+        pend -= full ? 8 : 0;
       }
+      a0 = b0;
+      a1 = b1;
+      b0 = cn.x;
+      b1 = cn.y;
+      ++ci;
     }
-    // the k-th emitted byte sits at lane_end - k; the last `pending` bytes are
-    // the low bytes of acc, most recent lowest
-    nbytes = stored_words * 4 + pending;
-    for (int j = 1; j <= pending; ++j) {
-      const int pos = stored_words * 4 + j;
-      if (pos <= kLaneBytes) lane_end[-pos] = (uint8_t)(acc >> (8 * (pending - j)));
+    // the last pend (< 8) bytes: byte i of the queue (i = 0 most recent) is
+    // emitted byte 8 blocks + pend - 1 - i, stored at lane_end - 1 - that
+    nbytes = 8 * blocks + pend;
+    for (int i = 0; i < pend; ++i) {
+      const int e_i = 8 * blocks + pend - 1 - i;
+      if (e_i < kLaneBytes) lane_end[-1 - e_i] = (uint8_t)(qlo >> (8 * i));
     }
     if (nbytes > kLaneBytes) set_status(st, kStatusLaneOverflow);
     recs[s].states[r] = x;
@@ -1274,7 +1291,7 @@ static void encode_pass(const EncodeBuffers& b, hipStream_t stream) {
     FVH_TIMED("k_rans", stream);
     hipLaunchKernelGGL((k_rans<FT>), dim3((b.shard_total + kRansShards - 1) / kRansShards), dim3(64), 0, stream,
                        b.frames, b.states, b.tiles, b.shards, b.shard_total, folded, b.recs,
-                       b.scratch);
+                       b.scratch, b.sink);
   }
 }
 
