@@ -28,6 +28,18 @@ sys.path.insert(0, str(ROOT))
 
 WIDTH, HEIGHT, FRAMES, QUALITY, TILE = 3840, 2160, 16, 90, (256, 128)
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = "pmc_traffic_4k_rgb8_q90.md"
+# library timing names -> rocprofv3 kernel names of this workload (16-bit residuals)
+PROFILE_NAMES = {
+    "k_band_enc": "k_band<true,true,u16>",
+    "k_band_dec": "k_band<false,true,u16>",
+    "k_analyze": "k_analyze<u16>",
+    "k_rans": "k_rans<u16>",
+    "k_emit": "k_emit<u16>",
+    "k_dec_o0": "k_dec_o0<u16>",
+    "k_dec_other": "k_dec_other<u16>",
+    "k_o0_parse": "k_o0_parse",
+}
 
 
 def parse_args():
@@ -40,6 +52,11 @@ def parse_args():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (the "
+                         "reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--traffic", default=str(ROOT / "profiles" / TRAFFIC_FILE),
+                    help="traffic summary written by fastvid_amd.prof_summary")
     return ap.parse_args()
 
 
@@ -116,6 +133,14 @@ def main():
     frames = [tuple(torch.from_numpy(p.astype(np.int32)).to(torch.uint16).to(device) for p in f)
               for f in host_frames]
     torch.cuda.synchronize()
+    if args.copy_calibration_mb > 0:
+        n = args.copy_calibration_mb << 20
+        src = torch.ones(n, dtype=torch.uint8, device=device)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     fargs = [frame_args(f, layout="rgb444", bit_depth=8, quality=QUALITY, tile_size=TILE)
              for f in frames]
 
@@ -224,17 +249,28 @@ def main():
             entry["gbs"] = round(bytes_per_launch / (per / 1e3) / 1e9, 1)
             entry["hbm_frac"] = round(entry["gbs"] / HBM_PEAK_GBS, 4)
         stages[name] = entry
+    from fastvid_amd.prof_summary import read_traffic
+    measured = read_traffic(args.traffic)
+
+    def traffic_of(name):
+        t = measured.get(PROFILE_NAMES.get(name, name))
+        return None if t is None else int(t[0] + t[1])
+
     dominant = max(kernels.items(), key=lambda kv: kv[1][1])[0] if kernels else None
     roofline = None
     if dominant in alg:
         d = stages[dominant]
+        launches_per_step = kernels[dominant][0] / args.steps
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": d["gbs"], "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": d["hbm_frac"], "traffic": None}
+                    "unit": "GB/s", "frac": d["hbm_frac"], "traffic": traffic_of(dominant),
+                    "algorithmic_bytes": int(alg[dominant] / launches_per_step)}
     predictor = None
     if "k_band_enc" in stages and "gbs" in stages["k_band_enc"]:
         d = stages["k_band_enc"]
+        launches_per_step = kernels["k_band_enc"][0] / args.steps
         predictor = {"kernel": "k_band_enc", "achieved": d["gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": d["hbm_frac"]}
+                     "unit": "GB/s", "frac": d["hbm_frac"], "traffic": traffic_of("k_band_enc"),
+                     "algorithmic_bytes": int(alg["k_band_enc"] / launches_per_step)}
 
     # ---- CPU baseline: the oracle on host cores, bounded sample
     cpu = None

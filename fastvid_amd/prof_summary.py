@@ -1,0 +1,129 @@
+"""Summaries of rocprofv3 CSV output as short markdown tables.
+
+    python -m fastvid_amd.prof_summary stats KERNEL_STATS_CSV --title T --cmd C
+        per-kernel calls, total and average duration from --kernel-trace --stats
+
+    python -m fastvid_amd.prof_summary traffic FETCH_CSV WRITE_CSV --copy-bytes N --title T --cmd C
+        per-kernel FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two --pmc runs,
+        each scaled by what the same counter reads for the largest copy kernel
+        dispatch of the run, a device-to-device copy of N bytes (so the table
+        reports HBM bytes per dispatch in units of a copy's bytes)
+
+The traffic table ends with one machine-readable line per kernel,
+``traffic: <kernel> <fetch_bytes> <write_bytes>``, which bench.py reads to fill
+its roofline's ``traffic`` field.
+"""
+
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import re
+import sys
+
+
+def short_name(name: str) -> str:
+    """'void fvh::k_band<true, true, unsigned short>(...)' -> 'k_band<true,true,u16>'."""
+    base = name.split("(")[0]
+    base = base.replace("void ", "").replace("fvh::", "").strip()
+    base = base.replace("unsigned short", "u16").replace("unsigned int", "u32").replace(" ", "")
+    return base
+
+
+def stats_table(path: str) -> list[str]:
+    rows = list(csv.DictReader(open(path)))
+    out = ["| kernel | calls | total ms | avg us | min us | max us | % |",
+           "|---|---:|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        out.append("| %s | %s | %.3f | %.1f | %.1f | %.1f | %s |" % (
+            short_name(r["Name"]), r["Calls"], float(r["TotalDurationNs"]) / 1e6,
+            float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3,
+            r["Percentage"]))
+    return out
+
+
+def per_dispatch(path: str, counter: str):
+    """{kernel: [value per dispatch]} and {dispatch: (kernel, grid)}."""
+    sums = collections.defaultdict(float)
+    info = {}
+    for r in csv.DictReader(open(path)):
+        if not r["Counter_Name"].startswith(counter):
+            continue
+        d = r["Dispatch_Id"]
+        sums[d] += float(r["Counter_Value"])
+        info[d] = (short_name(r["Kernel_Name"]), int(r["Grid_Size"]))
+    by_kernel = collections.defaultdict(list)
+    for d, v in sums.items():
+        by_kernel[info[d][0]].append(v)
+    return by_kernel, sums, info
+
+
+def copy_factor(sums, info, copy_bytes: int) -> float:
+    """Counter KiB of the largest copy dispatch divided by the copy's KiB."""
+    copies = [(info[d][1], sums[d]) for d in sums if "copy" in info[d][0].lower()]
+    if not copies:
+        raise SystemExit("no copy kernel dispatch in the run")
+    grid, value = max(copies)
+    return value / (copy_bytes / 1024.0)
+
+
+def traffic_table(fetch_csv: str, write_csv: str, copy_bytes: int) -> list[str]:
+    fk, fs, fi = per_dispatch(fetch_csv, "FETCH_SIZE")
+    wk, ws, wi = per_dispatch(write_csv, "WRITE_SIZE")
+    ff = copy_factor(fs, fi, copy_bytes)
+    wf = copy_factor(ws, wi, copy_bytes)
+    out = [f"Calibration: a {copy_bytes} B device-to-device copy reads {ff:.4f} x and writes "
+           f"{wf:.4f} x its size in FETCH_SIZE / WRITE_SIZE KiB; values below are divided by "
+           "these factors.", "",
+           "| kernel | dispatches | FETCH KiB/dispatch | WRITE KiB/dispatch | HBM MB/dispatch |",
+           "|---|---:|---:|---:|---:|"]
+    lines = []
+    for k in sorted(set(fk) | set(wk)):
+        f = sum(fk.get(k, [0])) / max(1, len(fk.get(k, [])))
+        w = sum(wk.get(k, [0])) / max(1, len(wk.get(k, [])))
+        fb = f * 1024 / ff
+        wb = w * 1024 / wf
+        out.append("| %s | %d | %.0f | %.0f | %.2f |" % (k, len(fk.get(k, [])), f, w,
+                                                          (fb + wb) / 1e6))
+        lines.append("traffic: %s %d %d" % (k, int(fb), int(wb)))
+    return out + ["", "```"] + lines + ["```"]
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="mode", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_csv")
+    t.add_argument("write_csv")
+    t.add_argument("--copy-bytes", type=int, required=True)
+    for p in (s, t):
+        p.add_argument("--title", default="rocprofv3 summary")
+        p.add_argument("--cmd", default="")
+    a = ap.parse_args(argv)
+    out = [f"# {a.title}", ""]
+    if a.cmd:
+        out += ["Command:", "", "```", a.cmd, "```", ""]
+    out += stats_table(a.csv) if a.mode == "stats" else \
+        traffic_table(a.fetch_csv, a.write_csv, a.copy_bytes)
+    print("\n".join(out))
+    return 0
+
+
+def read_traffic(path: str) -> dict:
+    """{kernel short name: (fetch_bytes, write_bytes)} from a traffic summary."""
+    table = {}
+    try:
+        for line in open(path):
+            m = re.match(r"traffic: (\S+) (\d+) (\d+)", line.strip())
+            if m:
+                table[m.group(1)] = (int(m.group(2)), int(m.group(3)))
+    except OSError:
+        pass
+    return table
+
+
+if __name__ == "__main__":
+    sys.exit(main())
