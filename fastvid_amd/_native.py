@@ -83,6 +83,9 @@ def lib():
     handle.fvh_decode_plan.argtypes = [p, i32, u64p, cs, sz]
     handle.fvh_decode_launch.restype = ctypes.c_int
     handle.fvh_decode_launch.argtypes = [p, p, p, i32, p, ctypes.c_uint64, p, i32, p, cs, sz]
+    handle.fvh_decode_launch_sized.restype = ctypes.c_int
+    handle.fvh_decode_launch_sized.argtypes = [p, p, p, p, i32, p, ctypes.c_uint64, p, i32, p,
+                                               cs, sz]
     handle.fvh_timing_enable.restype = ctypes.c_int
     handle.fvh_timing_enable.argtypes = [i32]
     handle.fvh_timing_read.restype = ctypes.c_int

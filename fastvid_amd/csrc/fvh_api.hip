@@ -19,19 +19,6 @@ void set_msg(char* msg, size_t len, const char* text) {
   if (msg && len) std::snprintf(msg, len, "%s", text);
 }
 
-int32_t quantization_step(int64_t layout, int64_t bit_depth, int64_t quality) {
-  // encode.cu:71-85
-  if ((layout == 0 && bit_depth == 8) || (layout == 1 && bit_depth == 8)) return 1;
-  int32_t denominator = 6;
-  if (layout == 1 && bit_depth == 10) denominator = 20;
-  else if (layout == 1 && bit_depth == 16) denominator = 24;
-  else if (layout == 2 && bit_depth == 10) denominator = 10;
-  const int32_t scale = int32_t{1} << (bit_depth - 8);
-  return 1 + (int32_t)(((100 - quality) * scale + denominator - 1) / denominator);
-}
-
-int32_t refined_gray16_step(int64_t quality) { return 1 + (int32_t)(((100 - quality) * 256 + 7) / 8); }
-
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Worst-case bytes of one shard record of n symbols: the larger of the
@@ -128,7 +115,7 @@ Geometry fill_geometry(FrameDesc& f, int64_t width, int64_t height, int64_t tw, 
 
 struct WorkspaceLayout {
   int64_t frames, states, tiles, shards, recs, tile_len, tile_off, folded, scratch, recon, dshards,
-      sink;
+      sink, queues;
   int64_t total;
 };
 
@@ -292,6 +279,7 @@ int build_encode_plan(const fvh_encode_frame* in, int32_t count, EncodePlan& pla
   w.recon = plan.any_wide ? take(plan.samples * 2) : -1;
   w.dshards = -1;
   w.sink = take(kSinkBytes);
+  w.queues = take(kQueueCount * 4);
   w.total = off;
   return FVH_OK;
 }
@@ -343,6 +331,7 @@ int fvh_encode_launch(const fvh_encode_frame* frames, int32_t frame_count, void*
   b.scratch = ws + plan.ws.scratch;
   b.recon = plan.ws.recon >= 0 ? (uint16_t*)(ws + plan.ws.recon) : nullptr;
   b.sink = ws + plan.ws.sink;
+  b.queues = (int32_t*)(ws + plan.ws.queues);
   b.results = (Results*)results;
   b.frame_count = frame_count;
   b.tile_total = (int)plan.tiles;
@@ -493,7 +482,7 @@ int fvh_decode_check_host(const uint8_t* bytes, int64_t size, const fvh_stream_i
 }
 
 struct DecodePlanLayout {
-  int64_t frames, states, tiles, shards, dshards, lists, o0tabs, folded, sink, total;
+  int64_t frames, states, tiles, shards, dshards, lists, o0tabs, folded, sink, queues, total;
   int64_t tiles_n, shards_n, samples;
   bool wide_folded, any_wide;
 };
@@ -528,6 +517,7 @@ static DecodePlanLayout decode_layout(const fvh_stream_info* infos, int32_t coun
   L.o0tabs = take(L.shards_n * (int64_t)sizeof(O0Tab));
   L.folded = take((L.samples + 8) * (L.wide_folded ? 4 : 2));
   L.sink = take(kSinkBytes);
+  L.queues = take(kQueueCount * 4);
   L.total = off;
   return L;
 }
@@ -547,10 +537,15 @@ int fvh_decode_plan(const fvh_stream_info* infos, int32_t count, uint64_t* works
   return FVH_OK;
 }
 
-int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
-                      void* const* outputs, int32_t count, void* workspace, uint64_t workspace_bytes,
-                      fvh_result* results, int32_t wavefront, fvh_stream_t stream, char* msg,
-                      size_t msg_len) {
+}  // extern "C"
+
+namespace fvh {
+namespace {
+
+int decode_launch(const fvh_stream_info* infos, const void* const* streams,
+                  const int64_t* fed_sizes, void* const* outputs, int32_t count, void* workspace,
+                  uint64_t workspace_bytes, fvh_result* results, int32_t wavefront,
+                  fvh_stream_t stream, char* msg, size_t msg_len) {
   if (count <= 0) {
     set_msg(msg, msg_len, "no streams to decode");
     return FVH_EARG;
@@ -583,8 +578,15 @@ int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
     f.band = band_eligible(f, outs);
     any_band |= f.band != 0;
     any_nonband |= f.band == 0;
-    any_med |= h.use_med != 0;
-    any_grad |= h.use_med == 0;
+    if (fed_sizes != nullptr) {
+      // the predictor comes from each stream's header on the device: gray
+      // frames may use either
+      any_med |= h.layout != 0 || h.bit_depth == 10;
+      any_grad |= h.layout == 0;
+    } else {
+      any_med |= h.use_med != 0;
+      any_grad |= h.use_med == 0;
+    }
     f.tile_begin = (int32_t)tiles;
     f.tile_count = (int32_t)g.tiles;
     f.shard_begin = (int32_t)shards;
@@ -610,6 +612,7 @@ int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
   b.list_count = b.lists + 2 * L.shards_n;
   b.o0tabs = (O0Tab*)(ws + L.o0tabs);
   b.sink = ws + L.sink;
+  b.queues = (int32_t*)(ws + L.queues);
   b.folded = ws + L.folded;
   b.results = (Results*)results;
   b.frame_count = count;
@@ -622,6 +625,7 @@ int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
   b.any_nonband = any_nonband;
   b.any_med = any_med;
   b.any_grad = any_grad;
+  b.fed_sizes = fed_sizes;
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
   launch_upload_frames(frames.data(), count, b.frames, s);
@@ -632,6 +636,31 @@ int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
     return FVH_EHIP;
   }
   return FVH_OK;
+}
+
+}  // namespace
+}  // namespace fvh
+
+extern "C" {
+
+int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
+                      void* const* outputs, int32_t count, void* workspace, uint64_t workspace_bytes,
+                      fvh_result* results, int32_t wavefront, fvh_stream_t stream, char* msg,
+                      size_t msg_len) {
+  return fvh::decode_launch(infos, streams, nullptr, outputs, count, workspace, workspace_bytes,
+                            results, wavefront, stream, msg, msg_len);
+}
+
+int fvh_decode_launch_sized(const fvh_stream_info* infos, const void* const* streams,
+                            const int64_t* stream_sizes, void* const* outputs, int32_t count,
+                            void* workspace, uint64_t workspace_bytes, fvh_result* results,
+                            int32_t wavefront, fvh_stream_t stream, char* msg, size_t msg_len) {
+  if (stream_sizes == nullptr) {
+    set_msg(msg, msg_len, "stream_sizes must be a device pointer");
+    return FVH_EARG;
+  }
+  return fvh::decode_launch(infos, streams, stream_sizes, outputs, count, workspace,
+                            workspace_bytes, results, wavefront, stream, msg, msg_len);
 }
 
 const char* fvh_build_info(void) { return "fastvid_hip 0.1 gfx950"; }

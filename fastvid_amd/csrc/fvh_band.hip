@@ -175,24 +175,34 @@ __device__ __forceinline__ int med3_asm(int a, int b, int c) {
 // groups) ahead into eight register slots, one per phase of the period, and
 // written to the line buffer in the group the line starts.  The band period
 // P is rounded up to whole lines so line starts stay on that schedule.
+constexpr int kRowChunks = 9;  // line buffer row: 8 chunks + 1 of padding against bank conflicts
+
+template <typename InT, typename OutT>
+struct BandLds {
+  v4u lin[4][kBandRows][kRowChunks * Vec8<InT>::kQuads];
+  v4u lout[4][kBandRows][kRowChunks * Vec8<OutT>::kQuads];
+  uint4 ring[4][kBandRingMax / 8];
+};
+
+// One tile group (four tiles, one per 16-lane row of the wave).
 template <bool kEncode, bool kMed, typename FT>
-__global__ __launch_bounds__(64) void k_band(const FrameDesc* __restrict__ frames,
-                                             FrameState* __restrict__ states,
-                                             const TileDesc* __restrict__ tiles, int tile_total,
-                                             FT* __restrict__ folded, char* gbase,
-                                             uint8_t* __restrict__ sink) {
+__device__ __forceinline__ void band_group(
+    const FrameDesc* __restrict__ frames, FrameState* __restrict__ states,
+    const TileDesc* __restrict__ tiles, int tile_total, FT* __restrict__ folded, char* gbase,
+    uint8_t* __restrict__ sink, int tile_group,
+    BandLds<typename std::conditional<kEncode, uint16_t, FT>::type,
+            typename std::conditional<kEncode, FT, uint16_t>::type>& lds) {
   using InT = typename std::conditional<kEncode, uint16_t, FT>::type;
   using OutT = typename std::conditional<kEncode, FT, uint16_t>::type;
   constexpr int kQi = Vec8<InT>::kQuads;
   constexpr int kQo = Vec8<OutT>::kQuads;
-  constexpr int kRowChunks = 9;  // 8 chunks + 1 of padding against bank conflicts
-  __shared__ v4u lin_all[4][kBandRows][kRowChunks * kQi];
-  __shared__ v4u lout_all[4][kBandRows][kRowChunks * kQo];
-  __shared__ uint4 ring_all[4][kBandRingMax / 8];
+  auto& lin_all = lds.lin;
+  auto& lout_all = lds.lout;
+  auto& ring_all = lds.ring;
   const int lane = lane_id();
   const int grp = lane >> 4;
   const int i = lane & 15;
-  const int t_idx = blockIdx.x * 4 + grp;
+  const int t_idx = tile_group * 4 + grp;
   uint4* ring = ring_all[grp];
   v4u (*lin)[kRowChunks * kQi] = lin_all[grp];
   v4u (*lout)[kRowChunks * kQo] = lout_all[grp];
@@ -390,29 +400,60 @@ __global__ __launch_bounds__(64) void k_band(const FrameDesc* __restrict__ frame
   }
 }
 
+// Persistent form: each wave takes tile groups from a device counter until
+// none are left, so frames of uneven size in one batch keep every wave busy
+// to the end.  The counter is zeroed by the call's state-initialisation
+// kernel.
+template <bool kEncode, bool kMed, typename FT>
+__global__ __launch_bounds__(64) void k_band(const FrameDesc* __restrict__ frames,
+                                             FrameState* __restrict__ states,
+                                             const TileDesc* __restrict__ tiles, int tile_total,
+                                             FT* __restrict__ folded, char* gbase,
+                                             uint8_t* __restrict__ sink,
+                                             int32_t* __restrict__ queue) {
+  using InT = typename std::conditional<kEncode, uint16_t, FT>::type;
+  using OutT = typename std::conditional<kEncode, FT, uint16_t>::type;
+  __shared__ BandLds<InT, OutT> lds;
+  const int groups = (tile_total + 3) / 4;
+  for (;;) {
+    int g = 0;
+    if (lane_id() == 0) g = atomicAdd(queue, 1);
+    g = __builtin_amdgcn_readfirstlane(g);
+    if (g >= groups) break;
+    band_group<kEncode, kMed, FT>(frames, states, tiles, tile_total, folded, gbase, sink, g, lds);
+  }
+}
+
 template <bool kEncode, typename FT>
 void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* tiles, int tile_total,
-                 FT* folded, uint8_t* sink, bool med, bool plain, hipStream_t stream) {
-  const dim3 grid((tile_total + 3) / 4);
+                 FT* folded, uint8_t* sink, int32_t* queues, bool med, bool plain,
+                 hipStream_t stream) {
+  // enough waves to fill the device: eight per compute unit
+  const int groups = (tile_total + 3) / 4;
+  const dim3 grid(min(groups, 8 * device_cus()));
   if (med) {
     FVH_TIMED(kEncode ? "k_band_enc" : "k_band_dec", stream);
     hipLaunchKernelGGL((k_band<kEncode, true, FT>), grid, dim3(64), 0, stream, frames, states,
-                       tiles, tile_total, folded, (char*)folded, sink);
+                       tiles, tile_total, folded, (char*)folded, sink, queues);
   }
   if (plain) {
     FVH_TIMED(kEncode ? "k_band_enc_grad" : "k_band_dec_grad", stream);
     hipLaunchKernelGGL((k_band<kEncode, false, FT>), grid, dim3(64), 0, stream, frames, states,
-                       tiles, tile_total, folded, (char*)folded, sink);
+                       tiles, tile_total, folded, (char*)folded, sink, queues + 1);
   }
 }
 
 template void launch_band<true, uint16_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                          uint16_t*, uint8_t*, bool, bool, hipStream_t);
+                                          uint16_t*, uint8_t*, int32_t*, bool, bool,
+                                          hipStream_t);
 template void launch_band<true, uint32_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                          uint32_t*, uint8_t*, bool, bool, hipStream_t);
+                                          uint32_t*, uint8_t*, int32_t*, bool, bool,
+                                          hipStream_t);
 template void launch_band<false, uint16_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                           uint16_t*, uint8_t*, bool, bool, hipStream_t);
+                                           uint16_t*, uint8_t*, int32_t*, bool, bool,
+                                           hipStream_t);
 template void launch_band<false, uint32_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                           uint32_t*, uint8_t*, bool, bool, hipStream_t);
+                                           uint32_t*, uint8_t*, int32_t*, bool, bool,
+                                           hipStream_t);
 
 }  // namespace fvh

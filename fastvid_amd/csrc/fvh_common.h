@@ -49,6 +49,25 @@ constexpr int kStatusCapacity = 101;      // encoder: internal output bound viol
 constexpr int kStatusLaneOverflow = 102;  // encoder: rANS lane exceeded its bound
 constexpr int kStatusDirectory = 10;      // decoder: directory (decode.cu:131-186)
 constexpr int kStatusShardHeader = 11;    // decoder: shard walk (decode.cu:188-219)
+constexpr int kStatusHeader = 12;         // decoder, sizes on the device: a stream header that
+                                          // does not match the geometry the batch was planned for
+
+// quantization_step and refined_gray16_step (encode.cu:71-89), for host and
+// device: the device-fed decoder derives the step from each stream's header.
+__host__ __device__ inline int32_t quantization_step(int64_t layout, int64_t bit_depth,
+                                                     int64_t quality) {
+  if ((layout == 0 && bit_depth == 8) || (layout == 1 && bit_depth == 8)) return 1;
+  int32_t denominator = 6;
+  if (layout == 1 && bit_depth == 10) denominator = 20;
+  else if (layout == 1 && bit_depth == 16) denominator = 24;
+  else if (layout == 2 && bit_depth == 10) denominator = 10;
+  const int32_t scale = int32_t{1} << (bit_depth - 8);
+  return 1 + (int32_t)(((100 - quality) * scale + denominator - 1) / denominator);
+}
+
+__host__ __device__ inline int32_t refined_gray16_step(int64_t quality) {
+  return 1 + (int32_t)(((100 - quality) * 256 + 7) / 8);
+}
 
 struct FrameDesc {
   const uint16_t* plane[3];    // encode input planes (device)

@@ -24,16 +24,66 @@ __device__ __forceinline__ uint64_t rd_u64(const uint8_t* p) {
 }
 
 __global__ void k_dstate_init(FrameState* __restrict__ states, int frame_count,
-                              int32_t* __restrict__ list_count) {
+                              int32_t* __restrict__ list_count, int32_t* __restrict__ queues) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     list_count[0] = 0;
     list_count[1] = 0;
   }
+  if (i < kQueueCount) queues[i] = 0;
   if (i >= frame_count) return;
   FrameState s = {};
   s.active = 1;
   states[i] = s;
+}
+
+// Device-fed decode: the stream sizes are in device memory and each header is
+// read here, one thread per frame.  The header must describe the geometry the
+// batch was planned for (dimensions, tiles, layout, bit depth; checks of
+// fastvid_cuda.cpp:141-183 on what varies per stream); the quality and the
+// gray MED / refined-step flags then give the frame's step and predictor.
+__global__ void k_dheader(FrameDesc* __restrict__ frames, FrameState* __restrict__ states,
+                          int frame_count, const int64_t* __restrict__ sizes, char* gbase) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= frame_count) return;
+  FrameDesc& f = frames[i];
+  FrameState* st = &states[i];
+  const int64_t size = sizes[i];
+  if (size < kHeaderBytes) {
+    set_status(st, kStatusHeader);
+    return;
+  }
+  const uint8_t* b = as_global(gbase, (const uint8_t*)f.stream);
+  uint32_t w[8];
+  for (int k = 0; k < 8; ++k) {
+    w[k] = (uint32_t)b[4 * k] | ((uint32_t)b[4 * k + 1] << 8) | ((uint32_t)b[4 * k + 2] << 16) |
+           ((uint32_t)b[4 * k + 3] << 24);
+  }
+  const int version = (int)(w[1] & 0xff);
+  const int layout = (int)((w[1] >> 8) & 0xff);
+  const int quality = (int)((w[1] >> 16) & 0xff);
+  const int depth_byte = (int)(w[1] >> 24);
+  const int bit_depth = (depth_byte & 0x3f) + 8;
+  const bool refined = (depth_byte & 0x80) != 0;
+  const bool gray_med = (depth_byte & 0x40) != 0;
+  const bool gray = layout == 0;
+  bool ok = w[0] == ('F' | ('V' << 8) | ('I' << 16) | ((uint32_t)'D' << 24)) && version == 1 &&
+            layout == f.layout && bit_depth == f.bit_depth && quality >= 1 && quality <= 100 &&
+            (!refined || (gray && bit_depth == 16 && quality < 100)) &&
+            (!gray_med || (gray && bit_depth == 10 && quality < 100)) &&
+            (int64_t)w[2] == f.width && (int64_t)w[3] == f.height &&
+            (int32_t)(w[4] & 0xffff) == f.tile_w && (int32_t)(w[4] >> 16) == f.tile_h &&
+            w[5] != 0 && w[6] != 0 && (int32_t)w[7] == f.tile_count &&
+            kHeaderBytes + (int64_t)f.tile_count * kDirectoryEntryBytes <= size;
+  if (!ok) {
+    set_status(st, kStatusHeader);
+    return;
+  }
+  f.quality = quality;
+  f.step = refined ? refined_gray16_step(quality) : quantization_step(layout, bit_depth, quality);
+  f.use_med = gray_med || !gray;
+  f.header_flags = depth_byte & 0xc0;
+  f.stream_capacity = size;
 }
 
 // Directory entry and shard-header walk of one tile (decode.cu:139-219).
@@ -795,10 +845,15 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
   const dim3 fgrid((b.frame_count + 63) / 64);
   {
     FVH_TIMED("k_dstate_init", stream);
-    hipLaunchKernelGGL(k_dstate_init, fgrid, dim3(64), 0, stream, b.states, b.frame_count,
-                       b.list_count);
+    hipLaunchKernelGGL(k_dstate_init, dim3((max(b.frame_count, kQueueCount) + 63) / 64), dim3(64),
+                       0, stream, b.states, b.frame_count, b.list_count, b.queues);
   }
   launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.tile_total, stream);
+  if (b.fed_sizes != nullptr) {
+    FVH_TIMED("k_dheader", stream);
+    hipLaunchKernelGGL(k_dheader, fgrid, dim3(64), 0, stream, b.frames, b.states, b.frame_count,
+                       b.fed_sizes, (char*)b.folded);
+  }
   {
     FVH_TIMED("k_parse", stream);
     hipLaunchKernelGGL(k_parse, dim3((b.tile_total + 255) / 256), dim3(256), 0, stream, b.frames,

@@ -47,8 +47,9 @@ void launch_upload_frames(const FrameDesc* host_frames, int frame_count, FrameDe
 // ----------------------------------------------------------- pass control
 
 __global__ void k_state_init(const FrameDesc* __restrict__ frames, FrameState* __restrict__ states,
-                             int frame_count) {
+                             int frame_count, int32_t* __restrict__ queues) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kQueueCount) queues[i] = 0;
   if (i >= frame_count) return;
   FrameState s = {};
   s.step = frames[i].step0;
@@ -1254,11 +1255,11 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
 // ------------------------------------------------------------ orchestration
 
 template <typename FT>
-static void encode_pass(const EncodeBuffers& b, hipStream_t stream) {
+static void encode_pass(const EncodeBuffers& b, int pass, hipStream_t stream) {
   FT* folded = (FT*)b.folded;
   if (b.any_lossy && b.any_band) {
-    launch_band<true, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.sink, b.any_med,
-                          b.any_grad, stream);
+    launch_band<true, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.sink,
+                          b.queues + 2 * pass, b.any_med, b.any_grad, stream);
   }
   if (b.any_lossy && b.any_nonband) {
     {
@@ -1301,9 +1302,10 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
   const dim3 tgrid((b.tile_total + 3) / 4);
   {
     FVH_TIMED("k_state_init", stream);
-    hipLaunchKernelGGL(k_state_init, fgrid, dim3(64), 0, stream, b.frames, b.states, b.frame_count);
+    hipLaunchKernelGGL(k_state_init, dim3((max(b.frame_count, kQueueCount) + 63) / 64), dim3(64), 0,
+                       stream, b.frames, b.states, b.frame_count, b.queues);
   }
-  encode_pass<FT>(b, stream);
+  encode_pass<FT>(b, 0, stream);
   if (b.any_heuristic) {
     {
       FVH_TIMED("k_tile_scan", stream);
@@ -1319,7 +1321,7 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
       FVH_TIMED("k_decide", stream);
       hipLaunchKernelGGL(k_decide, fgrid, dim3(64), 0, stream, b.frames, b.states, b.frame_count, 1);
     }
-    encode_pass<FT>(b, stream);
+    encode_pass<FT>(b, 1, stream);
     if (b.any_gray16) {
       {
         FVH_TIMED("k_tile_scan", stream);
@@ -1336,7 +1338,7 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
         hipLaunchKernelGGL(k_decide, fgrid, dim3(64), 0, stream, b.frames, b.states, b.frame_count,
                            2);
       }
-      encode_pass<FT>(b, stream);
+      encode_pass<FT>(b, 2, stream);
     }
     {
       FVH_TIMED("k_decide", stream);
@@ -1372,8 +1374,8 @@ template <typename FT>
 void launch_reconstruct(const DecodeBuffers& b, hipStream_t stream) {
   FT* folded = (FT*)b.folded;
   if (b.any_band) {
-    launch_band<false, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.sink, b.any_med,
-                           b.any_grad, stream);
+    launch_band<false, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.sink, b.queues,
+                           b.any_med, b.any_grad, stream);
   }
   if (b.any_nonband) {
     FVH_TIMED("k_wavefront_dec", stream);

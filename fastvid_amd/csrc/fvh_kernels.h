@@ -9,6 +9,7 @@ constexpr int kRingMax = 4096;     // widest tile handled by the wave-per-tile w
 constexpr int kBandRingMax = 512;  // widest tile handled by the band wavefront
 constexpr int kBandSinkBytes = 32; // per-lane sink slot of the band wavefront (8 x uint32)
 constexpr int kSinkBytes = 256 * kBandSinkBytes;  // workspace sink region
+constexpr int kQueueCount = 8;     // work counters per call (two per predictor pass)
 
 struct EncodeBuffers {
   FrameDesc* frames;
@@ -22,6 +23,7 @@ struct EncodeBuffers {
   uint8_t* scratch;      // kScratchBytes per shard
   uint16_t* recon;       // tile-order reconstruction for tiles wider than kRingMax (or null)
   uint8_t* sink;         // kSinkBytes target of masked-off stores
+  int32_t* queues;       // kQueueCount work counters, zeroed by k_state_init
   Results* results;
   int frame_count;
   int tile_total;
@@ -48,6 +50,8 @@ struct DecodeBuffers {
   int32_t* list_count;   // [0] order-0, [1] others
   O0Tab* o0tabs;         // parsed order-0 tables, by order-0 list position
   uint8_t* sink;         // kSinkBytes target of masked-off stores
+  int32_t* queues;       // kQueueCount work counters, zeroed by k_dstate_init
+  const int64_t* fed_sizes;  // device stream sizes (headers then parsed on the device), or null
   void* folded;
   Results* results;
   int frame_count;
@@ -75,6 +79,7 @@ template <typename FT>
 void launch_reconstruct(const DecodeBuffers& b, hipStream_t stream);
 template <bool kEncode, typename FT>
 void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* tiles, int tile_total,
-                 FT* folded, uint8_t* sink, bool med, bool plain, hipStream_t stream);
+                 FT* folded, uint8_t* sink, int32_t* queues, bool med, bool plain,
+                 hipStream_t stream);
 
 }  // namespace fvh

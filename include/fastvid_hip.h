@@ -111,6 +111,21 @@ int fvh_decode_launch(const fvh_stream_info* infos, const void* const* streams,
                       uint64_t workspace_bytes, fvh_result* results, int32_t wavefront,
                       fvh_stream_t stream, char* msg, size_t msg_len);
 
+/* As fvh_decode_launch, with the stream sizes in device memory
+ * (stream_sizes[i], int64) and each header read on the device, so the call
+ * can be captured once in a graph and replayed for new streams of the same
+ * geometry (no counterpart in the reference, whose decode parses the header
+ * on the host, cuda/csrc/fastvid_cuda.cpp:141-183).  infos[i] gives the
+ * geometry the batch is planned for: width, height, tile size, layout and
+ * bit depth; quality, step, predictor and stream_size are taken from the
+ * stream.  streams[i] must hold at least stream_sizes[i] bytes.  A header
+ * that does not match its info yields status 12 (otherwise the statuses of
+ * fvh_decode_launch). */
+int fvh_decode_launch_sized(const fvh_stream_info* infos, const void* const* streams,
+                            const int64_t* stream_sizes, void* const* outputs, int32_t count,
+                            void* workspace, uint64_t workspace_bytes, fvh_result* results,
+                            int32_t wavefront, fvh_stream_t stream, char* msg, size_t msg_len);
+
 /* Per-kernel timing (no counterpart in the reference, which is timed from
  * outside with CUDA events around whole calls, scripts/evaluate.py:447-463).
  * While enabled, every kernel launch of the library is bracketed by two
