@@ -563,7 +563,8 @@ int decode_launch(const fvh_stream_info* infos, const void* const* streams,
     FrameDesc& f = frames[i];
     const Geometry g = fill_geometry(f, h.width, h.height, h.tile_width, h.tile_height, h.layout);
     f.stream = (uint8_t*)streams[i];
-    f.stream_capacity = h.stream_size;
+    // with device sizes, k_dheader sets the size once the header checks out
+    f.stream_capacity = fed_sizes != nullptr ? 0 : h.stream_size;
     uint16_t* base = (uint16_t*)outputs[i];
     f.out_plane[0] = base;
     f.out_plane[1] = h.layout ? base + h.y_samples : nullptr;

@@ -99,6 +99,7 @@ __global__ __launch_bounds__(256) void k_parse(const FrameDesc* __restrict__ fra
   const TileDesc td = tiles[t];
   const FrameDesc& f = frames[td.frame];
   FrameState* st = &states[td.frame];
+  if (st->status != 0) return;  // e.g. a header rejected by k_dheader
   const uint8_t* enc = as_global(gbase, (const uint8_t*)f.stream);
   const int64_t size = f.stream_capacity;
   const int local = t - f.tile_begin;
