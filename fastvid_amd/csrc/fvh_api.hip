@@ -115,7 +115,7 @@ Geometry fill_geometry(FrameDesc& f, int64_t width, int64_t height, int64_t tw, 
 
 struct WorkspaceLayout {
   int64_t frames, states, tiles, shards, recs, tile_len, tile_off, folded, scratch, recon, dshards,
-      sink, queues;
+      sink, queues, groups;
   int64_t total;
 };
 
@@ -129,14 +129,41 @@ int64_t tile_count_closed(int64_t width, int64_t height, int64_t tw, int64_t th,
 
 // Frames whose every tile fits the band wavefront's vector accesses: planes
 // 16-byte aligned, pitches, plane widths and nominal tile widths multiples of
-// 8 samples, tiles no wider than kBandRingMax.
+// 8 samples, tiles no wider than kBandRingMax, and every plane and every
+// 16 rows of pitch addressable with 31-bit byte offsets (the kernel drops
+// masked stores by pushing their offset past 2^31).
 int band_eligible(const FrameDesc& f, const void* const* planes) {
   for (int p = 0; p < f.plane_count; ++p) {
     if (((uintptr_t)planes[p] & 15) != 0) return 0;
     if (f.pitch[p] % 8 != 0 || f.plane_width[p] % 8 != 0 || f.ntw[p] % 8 != 0) return 0;
     if (std::min<int64_t>(f.ntw[p], f.plane_width[p]) > kBandRingMax) return 0;
+    const int64_t span = ((f.height - 1) * f.pitch[p] + f.plane_width[p]) * 2;
+    if (span >= (int64_t{1} << 31) || 32 * f.pitch[p] >= (int64_t{1} << 31)) return 0;
   }
   return 1;
+}
+
+// Band tile groups (4 consecutive tiles of one plane) of one frame, numbered
+// from `first`; returns the frame's group count.
+int64_t number_groups(FrameDesc& f, int64_t first) {
+  int64_t g = first;
+  for (int p = 0; p < 3; ++p) {
+    f.group_begin[p] = (int32_t)g;
+    g += (f.plane_tile_begin[p + 1] - f.plane_tile_begin[p] + 3) / 4;
+  }
+  f.group_begin[3] = (int32_t)g;
+  return g - first;
+}
+
+// Ring slots per tile the band wavefront needs for a frame: its band period
+// in column groups, max(whole 64-sample lines, 16).
+int band_ring_groups(const FrameDesc& f) {
+  int64_t r = 0;
+  for (int p = 0; p < f.plane_count; ++p) {
+    const int64_t w = std::min<int64_t>(f.ntw[p], f.plane_width[p]);
+    r = std::max<int64_t>(r, std::max<int64_t>(((w + 63) / 64) * 8, 16));
+  }
+  return (int)r;
 }
 
 struct EncodePlan {
@@ -147,6 +174,8 @@ struct EncodePlan {
   bool wide_folded = false, any_heuristic = false, any_gray16 = false;
   bool any_lossless = false, any_lossy = false, any_wide = false;
   bool any_band = false, any_nonband = false, any_med = false, any_grad = false;
+  int64_t groups = 0;
+  int band_ring = 16;
   WorkspaceLayout ws = {};
 };
 
@@ -221,6 +250,8 @@ int build_encode_plan(const fvh_encode_frame* in, int32_t count, EncodePlan& pla
     plan.samples = (plan.samples + 7) & ~int64_t{7};  // 16-byte aligned residual rows
     f.folded_base = plan.samples;
     f.band = band_eligible(f, fr.planes);
+    plan.groups += number_groups(f, plan.groups);
+    if (f.band) plan.band_ring = std::max(plan.band_ring, band_ring_groups(f));
     f.tile_begin = (int32_t)plan.tiles;
     f.tile_count = (int32_t)g.tiles;
     f.shard_begin = (int32_t)plan.shards;
@@ -280,6 +311,7 @@ int build_encode_plan(const fvh_encode_frame* in, int32_t count, EncodePlan& pla
   w.dshards = -1;
   w.sink = take(kSinkBytes);
   w.queues = take(kQueueCount * 4);
+  w.groups = take(plan.groups * 4);
   w.total = off;
   return FVH_OK;
 }
@@ -332,6 +364,9 @@ int fvh_encode_launch(const fvh_encode_frame* frames, int32_t frame_count, void*
   b.recon = plan.ws.recon >= 0 ? (uint16_t*)(ws + plan.ws.recon) : nullptr;
   b.sink = ws + plan.ws.sink;
   b.queues = (int32_t*)(ws + plan.ws.queues);
+  b.group_first = (int32_t*)(ws + plan.ws.groups);
+  b.group_total = (int)plan.groups;
+  b.band_ring = plan.band_ring;
   b.results = (Results*)results;
   b.frame_count = frame_count;
   b.tile_total = (int)plan.tiles;
@@ -482,8 +517,8 @@ int fvh_decode_check_host(const uint8_t* bytes, int64_t size, const fvh_stream_i
 }
 
 struct DecodePlanLayout {
-  int64_t frames, states, tiles, shards, dshards, lists, o0tabs, folded, sink, queues, total;
-  int64_t tiles_n, shards_n, samples;
+  int64_t frames, states, tiles, shards, dshards, lists, o0tabs, folded, sink, queues, groups, total;
+  int64_t tiles_n, shards_n, samples, groups_n;
   bool wide_folded, any_wide;
 };
 
@@ -496,6 +531,7 @@ static DecodePlanLayout decode_layout(const fvh_stream_info* infos, int32_t coun
     const Geometry g = fill_geometry(scratch_desc, infos[i].width, infos[i].height,
                                      infos[i].tile_width, infos[i].tile_height, infos[i].layout);
     L.samples = ((L.samples + 7) & ~int64_t{7}) + g.samples;
+    L.groups_n += number_groups(scratch_desc, L.groups_n);
     L.wide_folded |= infos[i].bit_depth == 16;
     const int64_t tw = infos[i].tile_width;
     const int64_t ctw = infos[i].layout == 1 ? (tw + 1) / 2 : tw;
@@ -518,6 +554,7 @@ static DecodePlanLayout decode_layout(const fvh_stream_info* infos, int32_t coun
   L.folded = take((L.samples + 8) * (L.wide_folded ? 4 : 2));
   L.sink = take(kSinkBytes);
   L.queues = take(kQueueCount * 4);
+  L.groups = take(L.groups_n * 4);
   L.total = off;
   return L;
 }
@@ -556,7 +593,8 @@ int decode_launch(const fvh_stream_info* infos, const void* const* streams,
     return FVH_EARG;
   }
   std::vector<FrameDesc> frames(count);
-  int64_t tiles = 0, shards = 0, samples = 0;
+  int64_t tiles = 0, shards = 0, samples = 0, groups = 0;
+  int band_ring = 16;
   bool any_band = false, any_nonband = false, any_med = false, any_grad = false;
   for (int32_t i = 0; i < count; ++i) {
     const fvh_stream_info& h = infos[i];
@@ -577,6 +615,8 @@ int decode_launch(const fvh_stream_info* infos, const void* const* streams,
     f.folded_base = samples;
     const void* outs[3] = {f.out_plane[0], f.out_plane[1], f.out_plane[2]};
     f.band = band_eligible(f, outs);
+    groups += number_groups(f, groups);
+    if (f.band) band_ring = std::max(band_ring, band_ring_groups(f));
     any_band |= f.band != 0;
     any_nonband |= f.band == 0;
     if (fed_sizes != nullptr) {
@@ -614,6 +654,9 @@ int decode_launch(const fvh_stream_info* infos, const void* const* streams,
   b.o0tabs = (O0Tab*)(ws + L.o0tabs);
   b.sink = ws + L.sink;
   b.queues = (int32_t*)(ws + L.queues);
+  b.group_first = (int32_t*)(ws + L.groups);
+  b.group_total = (int)groups;
+  b.band_ring = band_ring;
   b.folded = ws + L.folded;
   b.results = (Results*)results;
   b.frame_count = count;

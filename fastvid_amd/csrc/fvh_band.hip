@@ -1,24 +1,37 @@
 // fvh_band.hip -- throughput wavefront for the closed-loop predictor
 // (encoder) and the reconstruction (decoder) on gfx950.
 //
-// Mapping: a wave64 is four DPP rows of 16 lanes; each row owns one tile.
-// Inside a tile, rows are taken in bands of 16: lane i of the row owns tile
-// row 16*b + i of band b and runs 8 steps behind lane i-1.  Because the lag
-// is 8, every lane's column satisfies x == step (mod 8), so each lane moves 8
-// samples (16 bytes) per 8 steps with one vector load and one vector store,
-// and inside an 8-step group every sample sits at a compile-time register
-// position.  The "above" neighbour of lane i is lane i-1's reconstruction
-// from 8 steps earlier: one DPP row_shr:1 per sample, no LDS and no barrier.
-// Lane 0 of a row takes the previous band's last row from a per-row LDS ring
-// written by lane 15 one band period (>= 128 steps) earlier.  A tile of
-// W x H costs ceil(H/16) * max(W,128) + 120 steps for 16*W*H/16 samples per
-// lane, i.e. >= 94% lane utilisation on 256x128 tiles, against the
-// reference's W+H-1 barrier-separated antidiagonals with neighbours re-read
-// from global memory (cuda/csrc/encode.cu:150-192, cuda/csrc/decode.cu:572-605).
+// Mapping: a wave64 is four DPP rows of 16 lanes; each row owns one tile, and
+// the four tiles of a wave ("tile group") are consecutive tiles of one plane,
+// so every plane address is a 32-bit offset from one wave-uniform buffer
+// resource.  Inside a tile, rows are taken in bands of 16: lane i of the row
+// owns tile row 16*b + i of band b and runs 8 steps behind lane i-1.  Because
+// the lag is 8, every lane's column satisfies x == step (mod 8), so each lane
+// moves 8 samples per 8 steps with one 16-byte LDS read and write, and inside
+// an 8-step group every sample sits at a compile-time register position.  The
+// "above" neighbour of lane i is lane i-1's reconstruction from 8 steps
+// earlier: one DPP row_shr:1 move per sample, no barrier.  Lane 0 of a row
+// takes the previous band's last row from a per-tile LDS ring instead: the
+// DPP move keeps its "old" operand on lane 0, and that operand is the ring
+// word.  A tile of W x H costs about ceil(H/16) * max(W,128) + 128 steps for
+// W*H/16 samples per lane (>= 94% lane utilisation on 256x128 tiles), against
+// the reference's W+H-1 barrier-separated antidiagonals with neighbours
+// re-read from global memory (cuda/csrc/encode.cu:150-192,
+// cuda/csrc/decode.cu:572-605).
+//
+// Arithmetic per sample (reference: encode.cu:168-190, decode.cu:584-600):
+// reconstructions are kept in a biased domain v + beta (beta = -(h + K*d) in
+// the encoder, 0 in the decoder).  MED and the gradient are invariant under a
+// common shift, so the prediction comes out biased too and the quantiser's
+// non-negative numerator is simply n = x - p_beta.  The quotient is exact
+// (see quantize) and the reconstruction is med3(beta, max + beta,
+// p_beta + q*d).  Residual folding and the decoder's unfolding run two samples
+// per instruction in packed 16-bit arithmetic.
 //
 // Eligible frames (host flag FrameDesc::band): 16-byte aligned planes, every
 // tile width, tile origin, pitch and residual base a multiple of 8 samples,
-// tile width <= kBandRingMax.  Other frames use k_wavefront.
+// tile width <= kBandRingMax, planes and pitches small enough for 32-bit
+// buffer offsets.  Other frames use k_wavefront.
 #include <type_traits>
 
 #include "fvh_device.h"
@@ -28,18 +41,31 @@
 namespace fvh {
 
 constexpr int kBandRows = 16;
-constexpr int kBandLag = 8;
-constexpr int kBandSkew = kBandLag * (kBandRows - 1);  // 120
+constexpr int kRowChunks = 9;              // line buffer row: 8 chunks + 1 against bank conflicts
+constexpr uint32_t kOob = 0x80000000u;     // buffer offset past every resource: access dropped
+constexpr int kRsrcWord3 = 0x00020000;     // raw buffer, 32-bit dwords (gfx9 family)
 
-__device__ __forceinline__ int dpp_row_shr1(int v) {
-  // row_shr:1 (0x111): lane i of each 16-lane row receives lane i-1; lane 0
-  // receives 0 (bound_ctrl) and is overridden by the caller.
-  return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);
+using v4u = uint32_t __attribute__((ext_vector_type(4)));
+using i16x2 = short __attribute__((ext_vector_type(2)));
+using u16x2 = unsigned short __attribute__((ext_vector_type(2)));
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+
+// row_shr:1 (0x111) within each 16-lane row; lane 0 of a row has no source
+// and keeps `old` (bound_ctrl off).
+__device__ __forceinline__ int above_dpp(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, 0x111, 0xf, 0xf, false);
 }
 
-// Native 4 x u32 vector: plain loads and stores (HIP's uint4 wrapper is
-// copied with memcpy, which in the 32-bit case kept chunks in private memory).
-using v4u = uint32_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, kRsrcWord3);
+}
+
+// v_med3_i32: the compiler folds med3(lo, hi, v) into a min/max pair otherwise.
+__device__ __forceinline__ int med3_asm(int a, int b, int c) {
+  int r;
+  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
 
 template <typename FT>
 struct Vec8;  // eight residuals / samples of one lane
@@ -48,25 +74,22 @@ template <>
 struct Vec8<uint16_t> {
   v4u v;
   static constexpr int kQuads = 1;  // 16-byte quads per chunk
+  __device__ __forceinline__ uint32_t word(int k) const {
+    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+  }
   __device__ __forceinline__ uint32_t get(int j) const {
-    const uint32_t w = j < 2 ? v.x : j < 4 ? v.y : j < 6 ? v.z : v.w;
+    const uint32_t w = word(j >> 1);
     return (j & 1) ? (w >> 16) : (w & 0xffffu);
   }
-  __device__ __forceinline__ void load(const uint16_t* p) { v = *(const v4u*)p; }
-  __device__ __forceinline__ void store(uint16_t* p) const { *(v4u*)p = v; }
+  __device__ __forceinline__ uint32_t orall() const { return v.x | v.y | v.z | v.w; }
+  __device__ __forceinline__ void load(rsrc_t r, uint32_t off) {
+    v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  }
+  __device__ __forceinline__ void store(rsrc_t r, uint32_t off) const {
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0);
+  }
   __device__ __forceinline__ void get_q(const v4u* q) { v = q[0]; }
   __device__ __forceinline__ void put_q(v4u* q) const { q[0] = v; }
-  __device__ __forceinline__ void zero() { v = v4u{0, 0, 0, 0}; }
-  __device__ __forceinline__ void set(int j, uint32_t x) {
-    const int k = j >> 1;
-    uint32_t w = k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
-    if (j & 1) w = (w & 0xffffu) | (x << 16);
-    else w = x & 0xffffu;
-    if (k == 0) v.x = w;
-    else if (k == 1) v.y = w;
-    else if (k == 2) v.z = w;
-    else v.w = w;
-  }
 };
 
 template <>
@@ -79,13 +102,19 @@ struct Vec8<uint32_t> {
       case 4: return b.x; case 5: return b.y; case 6: return b.z; default: return b.w;
     }
   }
-  __device__ __forceinline__ void load(const uint32_t* p) {
-    a = *(const v4u*)p;
-    b = *(const v4u*)(p + 4);
+  __device__ __forceinline__ void set(int j, uint32_t x) {
+    switch (j) {
+      case 0: a.x = x; break; case 1: a.y = x; break; case 2: a.z = x; break; case 3: a.w = x; break;
+      case 4: b.x = x; break; case 5: b.y = x; break; case 6: b.z = x; break; default: b.w = x; break;
+    }
   }
-  __device__ __forceinline__ void store(uint32_t* p) const {
-    *(v4u*)p = a;
-    *(v4u*)(p + 4) = b;
+  __device__ __forceinline__ void load(rsrc_t r, uint32_t off) {
+    a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, 0);
+  }
+  __device__ __forceinline__ void store(rsrc_t r, uint32_t off) const {
+    __builtin_amdgcn_raw_buffer_store_b128(a, r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(b, r, off + 16, 0, 0);
   }
   __device__ __forceinline__ void get_q(const v4u* q) {
     a = q[0];
@@ -95,365 +124,469 @@ struct Vec8<uint32_t> {
     q[0] = a;
     q[1] = b;
   }
-  __device__ __forceinline__ void zero() {
-    a = v4u{0, 0, 0, 0};
-    b = a;
-  }
-  __device__ __forceinline__ void set(int j, uint32_t x) {
-    switch (j) {
-      case 0: a.x = x; break; case 1: a.y = x; break; case 2: a.z = x; break; case 3: a.w = x; break;
-      case 4: b.x = x; break; case 5: b.y = x; break; case 6: b.z = x; break; default: b.w = x; break;
-    }
-  }
 };
 
-// Exact closed-loop quantiser q = sign(r) * floor((|r| + step/2) / step)
-// (cuda/csrc/encode.cu:183-185) without abs or negation:
-//   r >= 0: q = floor((r + h) / d)
-//   r <  0: q = ceil((r - h) / d) = floor((r - h + d - 1) / d)
-// Adding K*d (K = ceil(max/d)) makes the numerator n non-negative, then
-// q = floor(n / d) - K.  uint16 residual batches (bit depth <= 12): n < 2^(bd+2)
-// and m = ceil(2^s / d) with s = bd + 2 + ceil(log2 d) keep n*m < 2^32, so one
-// 24-bit multiply and a shift divide exactly.  uint32 batches (16-bit frames):
-// n < 2^18, mul_hi by ceil(2^32 / d).  Both are checked over every residual of
-// every reachable step in tests/test_capi.py.
-struct QuantMagic {
+// Quantiser of the reference, q = sign(r) * floor((|r| + h) / d) with
+// h = d / 2 (encode.cu:183-185), from the non-negative numerator
+//   n = r + h + K*d - [r < 0 and d even],   K = ceil(max / d),
+// for which q = floor(n / d) - K (r < 0: floor((r + d - 1 - h) / d) =
+// -floor((h - r) / d); for odd d, d - 1 - h == h).  Bit depth <= 12
+// (n < 2^(bd+2)): with s = bd + 2 + ceil(log2 d) and m = ceil(2^s / d),
+// floor(n / d) = (n * m) >> s, and since n * m < 2^31 and K * 2^s <= 2^31,
+// q = (n * m - K * 2^s) >> s as an arithmetic shift of one 24-bit
+// multiply-add.  16-bit samples (n < 2^18): floor(n / d) = mul_hi(n, ceil(2^32 / d)).
+// Both are checked over every residual of every reachable step in
+// tests/test_capi.py.
+struct Quant {
+  int32_t d;
+  int32_t lo, hi;      // reconstruction clamp in the biased domain
+  int32_t offp;        // h + K*d
   uint32_t m;
   uint32_t s;
+  uint32_t negks;      // -(K << s) mod 2^32 (bit depth <= 12)
   int32_t K;
-  int32_t offp;
-  int32_t offn;
 };
 
 template <typename FT>
-__device__ __forceinline__ QuantMagic quant_params(int step, int bit_depth, int max_sample) {
-  QuantMagic q;
+__device__ __forceinline__ Quant quant_setup(int step, int max_sample, int bit_depth) {
+  Quant q;
+  q.d = step;
   const int h = step >> 1;
   q.K = (max_sample + step - 1) / step;
   q.offp = h + q.K * step;
-  q.offn = step - 1 - h + q.K * step;
+  q.lo = -q.offp;
+  q.hi = max_sample - q.offp;
   if (sizeof(FT) == 2) {
     const uint32_t c = step > 1 ? 32 - __builtin_clz((uint32_t)step - 1) : 0;
     q.s = (uint32_t)bit_depth + 2 + c;
     q.m = (uint32_t)(((1ull << q.s) + (uint64_t)step - 1) / (uint64_t)step);
+    q.negks = 0u - ((uint32_t)q.K << q.s);
   } else {
     q.s = 32;
-    q.m = step > 1 ? quant_magic(step) : 0u;
+    q.m = quant_magic(step);
+    q.negks = 0;
   }
   return q;
 }
 
-template <typename FT>
-__device__ __forceinline__ int quantize(int r, const QuantMagic& q) {
-  const uint32_t n = (uint32_t)(r + (r < 0 ? q.offn : q.offp));
-  uint32_t f;
-  if (sizeof(FT) == 2) f = __umul24(n, q.m) >> q.s;
-  else f = (uint32_t)(((uint64_t)n * q.m) >> 32);
-  return (int)f - q.K;
-}
-
-// v_med3_i32: the compiler folds med3(0, hi, v) into a min/max pair.
-__device__ __forceinline__ int med3_asm(int a, int b, int c) {
-  int r;
-  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+// v_mad_u32_u24 written out: given mul24(a, b) + (0 - c) the compiler emits a
+// multiply and a subtract.
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
   return r;
 }
 
-// kEncode: IN = uint16 source samples (row pitch), OUT = residuals FT (tile rows)
-// decode:  IN = residuals FT (tile rows),          OUT = uint16 samples (row pitch)
-//
+template <typename FT, bool kOdd>
+__device__ __forceinline__ int quantize(int n, const Quant& q) {
+  if (!kOdd) n -= (uint32_t)n < (uint32_t)q.offp ? 1 : 0;
+  if (sizeof(FT) == 2) return (int)mad_u24((uint32_t)n, q.m, q.negks) >> q.s;
+  return (int)__umulhi((uint32_t)n, q.m) - q.K;
+}
+
+template <bool kEncode, typename FT>
+struct BandTypes {
+  using In = typename std::conditional<kEncode, uint16_t, FT>::type;
+  using Out = typename std::conditional<kEncode, FT, uint16_t>::type;
+  static constexpr int kQi = Vec8<In>::kQuads;
+  static constexpr int kQo = Vec8<Out>::kQuads;
+  // equal widths: a lane's output chunk overwrites the input chunk it has
+  // just consumed, and one line buffer serves both directions
+  static constexpr bool kInPlace = kQi == kQo;
+  static constexpr int kLineQuads = 4 * kBandRows * kRowChunks * kQi;
+  static constexpr int kOutQuads = kInPlace ? 0 : 4 * kBandRows * kRowChunks * kQo;
+};
+
+// Dynamic LDS of one wave: line buffers, then the four rings.
+template <bool kEncode, typename FT>
+__host__ __device__ constexpr int band_lds_bytes(int ring_groups) {
+  return (BandTypes<kEncode, FT>::kLineQuads + BandTypes<kEncode, FT>::kOutQuads) * 16 +
+         4 * ring_groups * 32;
+}
+
 // Memory traffic goes through per-row line buffers in LDS.  A "line" is 64
 // samples of a tile row (8 chunks of 8 samples, one chunk per group), and a
 // lane crosses into a new line every 8 groups; because lane i runs i groups
-// behind lane 0, the rows that start (or finish) a line in a given group are
-// exactly two, i and i + 8 for i = group mod 8 (start) or (group + 1) mod 8
-// (finish).  So in every group the 16 lanes of a tile load one line for each
-// of the two starting rows (lane l: row i + 8 (l / 8), chunk l mod 8) and
+// behind lane 0, the rows that start (or finish) a line in group k of an
+// 8-group iteration are exactly two, k and k + 8 (start) or k + 1 and k + 9
+// mod 16 (finish).  So in every group the 16 lanes of a tile load one line for
+// each of the two starting rows (lane l: row k + 8 (l / 8), chunk l mod 8) and
 // store the two finished lines, and each lane exchanges only its own chunk
-// with LDS.  Each memory instruction then covers whole 64-sample row
-// segments instead of 16 separate rows.  Loads are issued one line period (8
-// groups) ahead into eight register slots, one per phase of the period, and
-// written to the line buffer in the group the line starts.  The band period
-// P is rounded up to whole lines so line starts stay on that schedule.
-constexpr int kRowChunks = 9;  // line buffer row: 8 chunks + 1 of padding against bank conflicts
-
-template <typename InT, typename OutT>
-struct BandLds {
-  v4u lin[4][kBandRows][kRowChunks * Vec8<InT>::kQuads];
-  v4u lout[4][kBandRows][kRowChunks * Vec8<OutT>::kQuads];
-  uint4 ring[4][kBandRingMax / 8];
-};
-
-// One tile group (four tiles, one per 16-lane row of the wave).
-template <bool kEncode, bool kMed, typename FT>
+// with LDS.  Loads are issued one iteration ahead into eight register slots,
+// one per group of the iteration.  The band period GP (in groups) is rounded
+// up to whole lines so line starts stay on that schedule, and all lines
+// loaded in one iteration start at the same local time 8 (it + 1 - half),
+// all lines stored in groups 0..6 at 8 (it - 1 - half): line positions are
+// tracked once per iteration, not per group.
+//
+// The ring: in every group every lane writes its 8 reconstructions to ring
+// slot max(cg, 0) of its tile, and lane 0 reads slot cg + 1 at the end of the
+// group.  Lane 0 of band b+1 reads slot x one group before it works on column
+// group x, (b+1)*GP + x - 1 groups into the tile; lane 15 of band b wrote it at
+// b*GP + 15 + x <= that time (GP >= 16), every other lane of band b earlier,
+// and every lane of band b+1 later (lane i at (b+1)*GP + i + x), so the slot
+// holds the previous band's last row exactly when it is read.  Slots are
+// reset to beta when a tile starts (band 0 reads them); writes clamped to slot
+// 0 happen only before lane 15 first writes it.
+template <bool kEncode, bool kMed, bool kOdd, typename FT>
 __device__ __forceinline__ void band_group(
-    const FrameDesc* __restrict__ frames, FrameState* __restrict__ states,
-    const TileDesc* __restrict__ tiles, int tile_total, FT* __restrict__ folded, char* gbase,
-    uint8_t* __restrict__ sink, int tile_group,
-    BandLds<typename std::conditional<kEncode, uint16_t, FT>::type,
-            typename std::conditional<kEncode, FT, uint16_t>::type>& lds) {
-  using InT = typename std::conditional<kEncode, uint16_t, FT>::type;
-  using OutT = typename std::conditional<kEncode, FT, uint16_t>::type;
-  constexpr int kQi = Vec8<InT>::kQuads;
-  constexpr int kQo = Vec8<OutT>::kQuads;
-  auto& lin_all = lds.lin;
-  auto& lout_all = lds.lout;
-  auto& ring_all = lds.ring;
+    const TileDesc* __restrict__ tiles, int t0, int ntiles, rsrc_t in_r, rsrc_t out_r,
+    uint32_t in_pitch_u, uint32_t out_pitch_u, int64_t fb0, int ring_groups, const Quant& qz,
+    int max_sample, FrameState* st, v4u* smem) {
+  using T = BandTypes<kEncode, FT>;
+  using InT = typename T::In;
+  using OutT = typename T::Out;
+  constexpr int kQi = T::kQi;
+  constexpr int kQo = T::kQo;
+  constexpr uint32_t kCi = 16 * kQi;  // bytes of one input chunk
+  constexpr uint32_t kCo = 16 * kQo;
   const int lane = lane_id();
   const int grp = lane >> 4;
   const int i = lane & 15;
-  const int t_idx = tile_group * 4 + grp;
-  uint4* ring = ring_all[grp];
-  v4u (*lin)[kRowChunks * kQi] = lin_all[grp];
-  v4u (*lout)[kRowChunks * kQo] = lout_all[grp];
+  const int half = i >> 3;
+  const int c = i & 7;
 
-  // ---- per row-group tile setup
-  bool has = t_idx < tile_total;
+  // ---- this row's tile
+  const bool has = grp < ntiles;
   TileDesc td = {};
-  int step = 1, max_sample = 0, bit_depth = 8;
-  FrameState* st = nullptr;
-  if (has) {
-    td = tiles[t_idx];
-    const FrameDesc& f = frames[td.frame];
-    st = &states[td.frame];
-    has = f.band != 0;
-    if (kEncode) {
-      has = has && st->active && st->step > 1 && (st->use_med != 0) == kMed;
-      step = st->step;
-    } else {
-      has = has && st->status == 0 && (f.use_med != 0) == kMed;
-      step = f.step;
-    }
-    max_sample = f.max_sample;
-    bit_depth = f.bit_depth;
-  }
+  if (has) td = tiles[t0 + grp];
   const int W = has ? td.width : 0;
   const int H = has ? td.height : 0;
   const int nb = (H + kBandRows - 1) / kBandRows;
-  const int GW = ((W + 63) & ~63) >> 3;          // column groups of a row, whole lines
+  const int GW = ((W + 63) & ~63) >> 3;   // column groups of a row, whole lines
   // band period in groups: whole lines, and at least the 16 groups after
-  // which lane 15 has written the column lane 0 of the next band reads
+  // which lane 15 has written the ring slots lane 0 of the next band reads
   const int GP = max(GW, kBandRows);
-  const int WG = W >> 3;                          // column groups holding samples
+  const int WG = W >> 3;                  // column groups holding samples
   const int my_groups = has ? (nb - 1) * GP + GW + kBandRows : 0;
-  const int total = (int)wave_max((uint32_t)my_groups);
+  // wave-uniform trip count: a scalar loop, so the memory waits can be exact
+  const int total = __builtin_amdgcn_readfirstlane((int)wave_max((uint32_t)my_groups));
   if (total == 0) return;
-  const uint32_t gp_magic = quant_magic(GP);      // floor(t / GP) = mulhi(t, magic)
+  const int iters = (total + 7) >> 3;
 
-  const QuantMagic qm = quant_params<FT>(step, bit_depth, max_sample);
-  // Loads without a line to fetch read the start of a valid buffer, stores
-  // without one go to the lane's sink slot: every group issues the same
-  // memory operations on every path.
-  const InT* in_base = (const InT*)folded;
-  OutT* out_base = (OutT*)sink;
-  OutT* const my_sink = (OutT*)(sink + lane * kBandSinkBytes);
-  int64_t in_pitch = 0, out_pitch = 0;
-  if (has) {
-    const FrameDesc& f = frames[td.frame];
-    if (kEncode) {
-      in_base = as_global(gbase, (const InT*)(f.plane[td.plane] + td.src_offset));
-      in_pitch = f.pitch[td.plane];
-      out_base = (OutT*)(folded + td.folded_base);
-      out_pitch = W;
-    } else {
-      in_base = (const InT*)(folded + td.folded_base);
-      in_pitch = W;
-      out_base = as_global(gbase, (OutT*)(f.out_plane[td.plane] + td.dst_offset));
-      out_pitch = f.plane_width[td.plane];
-    }
+  uint32_t in_off, out_off, in_pitch, out_pitch;
+  if (kEncode) {
+    in_off = (uint32_t)td.src_offset * 2u;
+    in_pitch = in_pitch_u;
+    out_off = (uint32_t)(td.folded_base - fb0) * (uint32_t)sizeof(FT);
+    out_pitch = (uint32_t)W * (uint32_t)sizeof(FT);
+  } else {
+    in_off = (uint32_t)(td.folded_base - fb0) * (uint32_t)sizeof(FT);
+    in_pitch = (uint32_t)W * (uint32_t)sizeof(FT);
+    out_off = (uint32_t)td.dst_offset * 2u;
+    out_pitch = out_pitch_u;
   }
-  // row and first column group of the line that tile row r has reached at
-  // local time t (groups since the row group started; lines start at t = 0
-  // mod 8): returns false where there is no such line
-  auto line_at = [&](int r, int t, int* row, int* cg) __attribute__((always_inline)) {
-    if (t < 0) return false;
-    const int bb = (int)__umulhi((uint32_t)t, gp_magic);
-    *cg = t - bb * GP;
-    *row = bb * kBandRows + r;
-    return bb < nb && *row < H;
-  };
-  // cooperative role of this lane: chunk c of a line of row `base + 8 * half`
-  const int c = i & 7;
-  const int half = i >> 3;
-  auto issue_load = [&](Vec8<InT>& slot, int G, int k) __attribute__((always_inline)) {
-    // line of row k + 8 half starting at group G + 8
-    const int r = k + 8 * half;
-    int row = 0, cg = 0;
-    const bool ok = line_at(r, G + 8 - r, &row, &cg) && cg + c < WG;
-    slot.load(in_base + (ok ? (int64_t)row * in_pitch + 8 * (cg + c) : 0));
-  };
 
-  // own lane state: cg = column group (negative before the row group starts),
+  // ---- LDS
+  v4u* const lin = smem + grp * (kBandRows * kRowChunks * kQi);
+  v4u* const lout =
+      T::kInPlace ? lin : smem + T::kLineQuads + grp * (kBandRows * kRowChunks * kQo);
+  v4u* const ring = smem + T::kLineQuads + T::kOutQuads + grp * ring_groups * 2;
+  v4u* const lin_w = lin + ((8 * half) * kRowChunks + c) * kQi;          // + k rows
+  const v4u* const lout_r = lout + ((8 * half) * kRowChunks + c) * kQo;  // + ((k+1)&7) rows
+  const v4u* const lin_row = lin + i * kRowChunks * kQi;
+  v4u* const lout_row = lout + i * kRowChunks * kQo;
+
+  const int beta = kEncode ? qz.lo : 0;
+  const int lo = kEncode ? qz.lo : 0;
+  const int hi = kEncode ? qz.hi : max_sample;
+
+  for (int idx = i; idx < ring_groups * 2; idx += kBandRows) {
+    ring[idx] = v4u{(uint32_t)beta, (uint32_t)beta, (uint32_t)beta, (uint32_t)beta};
+  }
+
+  // ---- line trackers: local time of the line as band bb and column group cg
+  int ld_cg = -8 * half, ld_bb = 0;        // lines loaded in the iteration before `it`
+  int st_cg = -8 * (1 + half), st_bb = 0;  // lines finishing in groups 0..6 of `it`
+  auto ld_base = [&]() __attribute__((always_inline)) {
+    return in_off + __umul24((uint32_t)(16 * ld_bb + 8 * half), in_pitch) +
+           (uint32_t)(ld_cg + c) * kCi;
+  };
+  uint32_t st_base = kOob;
+  int st_left = 0;
+  auto st_update = [&]() __attribute__((always_inline)) {
+    const bool ok = st_cg >= 0 && st_bb < nb && st_cg + c < WG;
+    st_left = H - 16 * st_bb - 8 * half;
+    st_base = ok ? out_off + __umul24((uint32_t)(16 * st_bb + 8 * half), out_pitch) +
+                       (uint32_t)(st_cg + c) * kCo
+                 : kOob;
+  };
+  st_update();
+
+  // Slot loads stay in slot order, here and in the loop, and every prologue
+  // load is preceded by a store the hardware drops (offset past the
+  // resource), as every loop load is preceded by a line store.  The memory
+  // counter then reads the same on entry as on the back edge, and the wait
+  // before a slot's first use is exact (otherwise the loop head waits for
+  // every load in flight).
+  Vec8<InT> s0, s1, s2, s3, s4, s5, s6, s7;
+  {
+    const uint32_t b = ld_base();
+    Vec8<OutT> none;
+    none.get_q(lout_r);
+    // (distinct dropped offsets, or they are merged as one store)
+    auto prime = [&](Vec8<InT>& slot, uint32_t off, uint32_t k) __attribute__((always_inline)) {
+      none.store(out_r, kOob + 64u * k);
+      slot.load(in_r, off);
+      __builtin_amdgcn_sched_barrier(0);
+    };
+    prime(s0, b, 0);
+    prime(s1, b + in_pitch, 1);
+    prime(s2, b + __umul24(2u, in_pitch), 2);
+    prime(s3, b + __umul24(3u, in_pitch), 3);
+    prime(s4, b + __umul24(4u, in_pitch), 4);
+    prime(s5, b + __umul24(5u, in_pitch), 5);
+    prime(s6, b + __umul24(6u, in_pitch), 6);
+    prime(s7, b + __umul24(7u, in_pitch), 7);
+  }
+
+  // ---- own lane state: cg = column group (negative before the row starts),
   // bo = band
   int cg = -i;
   int bo = 0;
-  int L = 0, UL = 0;
-  int r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0, r5 = 0, r6 = 0, r7 = 0;
-  int bad = 0;
-  uint4 rv = make_uint4(0, 0, 0, 0);
-  auto row_ok = [&](int band) __attribute__((always_inline)) {
-    return band < nb && band * kBandRows + i < H;
-  };
+  int L = beta, UL = beta;
+  int r0 = beta, r1 = beta, r2 = beta, r3 = beta, r4 = beta, r5 = beta, r6 = beta, r7 = beta;
+  v4u rv0 = ring[0], rv1 = ring[1];  // ring slot of the next group (used by lane 0)
+  uint32_t acc = 0;                  // encoder: OR of every active sample
 
-  Vec8<InT> s0, s1, s2, s3, s4, s5, s6, s7;
-  issue_load(s0, -8, 0);
-  issue_load(s1, -7, 1);
-  issue_load(s2, -6, 2);
-  issue_load(s3, -5, 3);
-  issue_load(s4, -4, 4);
-  issue_load(s5, -3, 5);
-  issue_load(s6, -2, 6);
-  issue_load(s7, -1, 7);
-
-  auto group = [&](Vec8<InT>& slot, int G, int k) __attribute__((always_inline)) {
+  auto group = [&](Vec8<InT>& slot, uint32_t ldb, int k) __attribute__((always_inline)) {
     // (1) the line starting now for rows k and k + 8 goes to the line buffer
-    slot.put_q(&lin[k + 8 * half][c * kQi]);
-    // (2) this lane's chunk
-    const bool act = cg >= 0 && cg < WG && row_ok(bo);
-    const int x = cg * kBandLag;
-    if (cg == 0) {
-      L = 0;
-      UL = 0;
-    }
+    slot.put_q(lin_w + k * kRowChunks * kQi);
+    // (2) this lane's chunk: column group cg, position cg mod 8 of its line
+    const int pos = (k - i) & 7;
     Vec8<InT> cur;
-    cur.get_q(&lin[i][(cg & 7) * kQi]);
-    Vec8<OutT> out;
-    out.zero();
+    cur.get_q(lin_row + pos * kQi);
+    if (cg == 0) {
+      L = beta;
+      UL = beta;
+    }
+    const int ringv[8] = {(int)rv0.x, (int)rv0.y, (int)rv0.z, (int)rv0.w,
+                          (int)rv1.x, (int)rv1.y, (int)rv1.z, (int)rv1.w};
     int rr[8] = {r0, r1, r2, r3, r4, r5, r6, r7};
+    Vec8<OutT> out;
+    if constexpr (kEncode) {
+      const bool act = (uint32_t)cg < (uint32_t)WG && 16 * bo + i < H;
+      acc |= (cur.v.x | cur.v.y | cur.v.z | cur.v.w) & (act ? ~0u : 0u);
+      int q[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int from_prev = dpp_row_shr1(rr[j]);
-      const uint32_t rw = j < 2 ? rv.x : j < 4 ? rv.y : j < 6 ? rv.z : rv.w;
-      const int ring_v = (int)((j & 1) ? (rw >> 16) : (rw & 0xffffu));
-      const int above = i == 0 ? ring_v : from_prev;
-      const int grad = L + above - UL;
-      const int pred = kMed ? median3(L, above, grad) : med3_asm(0, max_sample, grad);
-      int rec;
-      if (kEncode) {
-        const int sample = (int)cur.get(j);
-        bad |= act & (sample > max_sample);
-        const int q = quantize<FT>(sample - pred, qm);
-        rec = med3_asm(0, max_sample, pred + __mul24(q, step));
-        out.set(j, ((uint32_t)q << 1) ^ (uint32_t)(q >> 31));
-      } else {
-        const uint32_t fv = cur.get(j);
-        const int q = (int)(fv >> 1) ^ -(int)(fv & 1u);
-        rec = med3_asm(0, max_sample, pred + __mul24(q, step));
-        out.set(j, (uint32_t)rec);
+      for (int j = 0; j < 8; ++j) {
+        const int A = above_dpp(ringv[j], rr[j]);
+        const int g = L + A - UL;
+        const int p = kMed ? median3(L, A, g) : med3_asm(lo, hi, g);
+        q[j] = quantize<FT, kOdd>((int)cur.get(j) - p, qz);
+        const int rec = med3_asm(lo, hi, p + __mul24(q[j], qz.d));
+        rr[j] = rec;
+        L = rec;
+        UL = A;
       }
-      rr[j] = rec;
-      L = rec;
-      UL = above;
+      if constexpr (sizeof(FT) == 2) {
+        // zigzag of two residuals per instruction: (q << 1) ^ (q >> 15) on 16-bit halves
+        uint32_t w[4];
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) {
+          const i16x2 P = i16x2{(short)q[2 * k2], (short)q[2 * k2 + 1]};
+          const i16x2 F = (P << (short)1) ^ (P >> (short)15);
+          w[k2] = __builtin_bit_cast(uint32_t, F);
+        }
+        out.v = v4u{w[0], w[1], w[2], w[3]};
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) out.set(j, ((uint32_t)q[j] << 1) ^ (uint32_t)(q[j] >> 31));
+      }
+    } else {
+      int qd[8];
+      if constexpr (sizeof(FT) == 2) {
+        // unzigzag and scale two residuals per instruction (|q*d| < 2^15)
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) {
+          const u16x2 V = __builtin_bit_cast(u16x2, cur.word(k2));
+          const i16x2 Q = (i16x2)(V >> (unsigned short)1) ^ -(i16x2)(V & (unsigned short)1);
+          const i16x2 QD = Q * (short)qz.d;
+          qd[2 * k2] = QD.x;
+          qd[2 * k2 + 1] = QD.y;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const uint32_t fv = cur.get(j);
+          qd[j] = __mul24((int)(fv >> 1) ^ -(int)(fv & 1u), qz.d);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int A = above_dpp(ringv[j], rr[j]);
+        const int g = L + A - UL;
+        const int p = kMed ? median3(L, A, g) : med3_asm(lo, hi, g);
+        const int rec = med3_asm(lo, hi, p + qd[j]);
+        rr[j] = rec;
+        L = rec;
+        UL = A;
+      }
+      out.v = v4u{(uint32_t)rr[0] | ((uint32_t)rr[1] << 16), (uint32_t)rr[2] | ((uint32_t)rr[3] << 16),
+                   (uint32_t)rr[4] | ((uint32_t)rr[5] << 16), (uint32_t)rr[6] | ((uint32_t)rr[7] << 16)};
     }
     r0 = rr[0]; r1 = rr[1]; r2 = rr[2]; r3 = rr[3];
     r4 = rr[4]; r5 = rr[5]; r6 = rr[6]; r7 = rr[7];
-    out.put_q(&lout[i][(cg & 7) * kQo]);
-    if (act && i == kBandRows - 1 && bo + 1 < nb) {
-      ring[x >> 3] = make_uint4((uint32_t)r0 | ((uint32_t)r1 << 16),
-                                (uint32_t)r2 | ((uint32_t)r3 << 16),
-                                (uint32_t)r4 | ((uint32_t)r5 << 16),
-                                (uint32_t)r6 | ((uint32_t)r7 << 16));
-    }
-    // (3) store the lines finished now, rows k + 1 and k + 9 (mod 16)
+    out.put_q(lout_row + pos * kQo);
+    // (3) ring slot of this column group
     {
-      const int r = ((k + 1) & 7) + 8 * half;
-      int row = 0, lcg = 0;
-      // lcg is the line's last column group here (7 mod 8)
-      const bool ok = line_at(r, G - r, &row, &lcg) && lcg - 7 + c < WG;
-      Vec8<OutT> done;
-      done.get_q(&lout[r][c * kQo]);
-      int ok_store = ok;
-      asm volatile("" : "+v"(ok_store));
-      done.store(ok_store ? out_base + (int64_t)row * out_pitch + 8 * (lcg - 7 + c) : my_sink);
+      v4u* rp = ring + max(cg, 0) * 2;
+      rp[0] = v4u{(uint32_t)r0, (uint32_t)r1, (uint32_t)r2, (uint32_t)r3};
+      rp[1] = v4u{(uint32_t)r4, (uint32_t)r5, (uint32_t)r6, (uint32_t)r7};
     }
-    if (kEncode) asm volatile("" : "+v"(bad));
-    // (4) the line starting at G + 8 for rows k and k + 8, into the slot just
-    // emptied (after its last use, so the load can target the slot's registers)
+    // (4) store the lines finished now, rows k + 1 and k + 9 (mod 16)
+    {
+      uint32_t off;
+      if (k < 7) {
+        const int kk = k + 1;
+        off = kk < st_left ? st_base + __umul24((uint32_t)kk, out_pitch) : kOob;
+      } else {
+        st_cg += 8;
+        if (st_cg >= GP) {
+          st_cg -= GP;
+          ++st_bb;
+        }
+        st_update();
+        off = st_left > 0 ? st_base : kOob;
+      }
+      Vec8<OutT> done;
+      done.get_q(lout_r + ((k + 1) & 7) * kRowChunks * kQo);
+      done.store(out_r, off);
+    }
+    // (5) the line starting in the next iteration for rows k and k + 8, into
+    // the slot just emptied (pinned in slot order, see above)
     __builtin_amdgcn_sched_barrier(0);
-    issue_load(slot, G, k);
+    slot.load(in_r, ldb + __umul24((uint32_t)k, in_pitch));
+    __builtin_amdgcn_sched_barrier(0);
     // advance
     ++cg;
     if (cg >= GP) {
       cg = 0;
       ++bo;
     }
-    // ring words for the next group (written by lane 15 at least one group earlier)
-    rv = make_uint4(0, 0, 0, 0);
-    if (i == 0 && cg >= 0 && cg < WG && bo > 0 && row_ok(bo)) rv = ring[cg];
+    {
+      const v4u* rq = ring + max(cg, 0) * 2;
+      rv0 = rq[0];
+      rv1 = rq[1];
+    }
   };
 
-  for (int G = 0; G < total; G += 8) {
-    group(s0, G + 0, 0);
-    group(s1, G + 1, 1);
-    group(s2, G + 2, 2);
-    group(s3, G + 3, 3);
-    group(s4, G + 4, 4);
-    group(s5, G + 5, 5);
-    group(s6, G + 6, 6);
-    group(s7, G + 7, 7);
-  }
-  if (kEncode) {
-    const uint64_t any = ballot(bad != 0);
-    if (i == 0 && st != nullptr && ((any >> (grp * 16)) & 0xffffull) != 0) {
-      set_status(st, kStatusSampleRange);
+  // do-while: at least one iteration (total > 0), so no guard the prologue
+  // loads could be sunk past
+  int it = 0;
+  do {
+    ld_cg += 8;
+    if (ld_cg >= GP) {
+      ld_cg -= GP;
+      ++ld_bb;
     }
+    const uint32_t ldb = ld_base();
+    group(s0, ldb, 0);
+    group(s1, ldb, 1);
+    group(s2, ldb, 2);
+    group(s3, ldb, 3);
+    group(s4, ldb, 4);
+    group(s5, ldb, 5);
+    group(s6, ldb, 6);
+    group(s7, ldb, 7);
+  } while (++it < iters);
+  if constexpr (kEncode) {
+    const uint32_t mask = ~((uint32_t)max_sample | ((uint32_t)max_sample << 16));
+    const uint64_t any = ballot((acc & mask) != 0);
+    if (any != 0 && lane == 0) set_status(st, kStatusSampleRange);
   }
 }
 
 // Persistent form: each wave takes tile groups from a device counter until
 // none are left, so frames of uneven size in one batch keep every wave busy
 // to the end.  The counter is zeroed by the call's state-initialisation
-// kernel.
+// kernel.  A tile group's frame, plane and quantiser are wave-uniform.
 template <bool kEncode, bool kMed, typename FT>
 __global__ __launch_bounds__(64) void k_band(const FrameDesc* __restrict__ frames,
                                              FrameState* __restrict__ states,
-                                             const TileDesc* __restrict__ tiles, int tile_total,
-                                             FT* __restrict__ folded, char* gbase,
-                                             uint8_t* __restrict__ sink,
-                                             int32_t* __restrict__ queue) {
-  using InT = typename std::conditional<kEncode, uint16_t, FT>::type;
-  using OutT = typename std::conditional<kEncode, FT, uint16_t>::type;
-  __shared__ BandLds<InT, OutT> lds;
-  const int groups = (tile_total + 3) / 4;
+                                             const TileDesc* __restrict__ tiles,
+                                             const int32_t* __restrict__ group_first,
+                                             int group_total, FT* __restrict__ folded,
+                                             int ring_groups, int32_t* __restrict__ queue) {
+  extern __shared__ v4u band_smem[];
   for (;;) {
     int g = 0;
     if (lane_id() == 0) g = atomicAdd(queue, 1);
     g = __builtin_amdgcn_readfirstlane(g);
-    if (g >= groups) break;
-    band_group<kEncode, kMed, FT>(frames, states, tiles, tile_total, folded, gbase, sink, g, lds);
+    if (g >= group_total) break;
+    const int t0 = group_first[g];
+    const int fi = tiles[t0].frame;
+    const int p = tiles[t0].plane;
+    const FrameDesc& f = frames[fi];
+    if (!f.band) continue;
+    FrameState* st = states + fi;
+    int step;
+    if (kEncode) {
+      if (!st->active || st->step <= 1 || (st->use_med != 0) != kMed) continue;
+      step = st->step;
+    } else {
+      if (st->status != 0 || (f.use_med != 0) != kMed) continue;
+      step = f.step;
+    }
+    const int ntiles = min(4, f.tile_begin + f.plane_tile_begin[p + 1] - t0);
+    const int tl = t0 + ntiles - 1;
+    const int64_t fb0 = tiles[t0].folded_base;
+    const int64_t fb_end =
+        tiles[tl].folded_base + (((int64_t)tiles[tl].width * tiles[tl].height + 7) & ~int64_t{7});
+    const uint32_t res_bytes = (uint32_t)((fb_end - fb0) * (int64_t)sizeof(FT));
+    const Quant qz = quant_setup<FT>(step, f.max_sample, f.bit_depth);
+    rsrc_t in_r, out_r;
+    uint32_t in_pitch = 0, out_pitch = 0;
+    if (kEncode) {
+      const uint32_t plane_bytes =
+          (uint32_t)(((f.height - 1) * f.pitch[p] + f.plane_width[p]) * 2);
+      in_r = make_rsrc(f.plane[p], plane_bytes);
+      out_r = make_rsrc(folded + fb0, res_bytes);
+      in_pitch = (uint32_t)f.pitch[p] * 2u;
+    } else {
+      const uint32_t plane_bytes = (uint32_t)(f.height * f.plane_width[p] * 2);
+      in_r = make_rsrc(folded + fb0, res_bytes);
+      out_r = make_rsrc(f.out_plane[p], plane_bytes);
+      out_pitch = (uint32_t)f.plane_width[p] * 2u;
+    }
+    if (kEncode && (step & 1) == 0) {
+      band_group<kEncode, kMed, false, FT>(tiles, t0, ntiles, in_r, out_r, in_pitch, out_pitch,
+                                           fb0, ring_groups, qz, f.max_sample, st, band_smem);
+    } else {
+      band_group<kEncode, kMed, true, FT>(tiles, t0, ntiles, in_r, out_r, in_pitch, out_pitch,
+                                          fb0, ring_groups, qz, f.max_sample, st, band_smem);
+    }
   }
 }
 
 template <bool kEncode, typename FT>
-void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* tiles, int tile_total,
-                 FT* folded, uint8_t* sink, int32_t* queues, bool med, bool plain,
-                 hipStream_t stream) {
-  // enough waves to fill the device: eight per compute unit
-  const int groups = (tile_total + 3) / 4;
-  const dim3 grid(min(groups, 8 * device_cus()));
+void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* tiles,
+                 const int32_t* group_first, int group_total, int ring_groups, FT* folded,
+                 int32_t* queues, bool med, bool plain, hipStream_t stream) {
+  if (group_total <= 0) return;
+  const int lds = band_lds_bytes<kEncode, FT>(ring_groups);
+  // persistent waves: at most what the device holds at once
+  const dim3 grid(min(group_total, 16 * device_cus()));
   if (med) {
     FVH_TIMED(kEncode ? "k_band_enc" : "k_band_dec", stream);
-    hipLaunchKernelGGL((k_band<kEncode, true, FT>), grid, dim3(64), 0, stream, frames, states,
-                       tiles, tile_total, folded, (char*)folded, sink, queues);
+    hipLaunchKernelGGL((k_band<kEncode, true, FT>), grid, dim3(64), lds, stream, frames, states,
+                       tiles, group_first, group_total, folded, ring_groups, queues);
   }
   if (plain) {
     FVH_TIMED(kEncode ? "k_band_enc_grad" : "k_band_dec_grad", stream);
-    hipLaunchKernelGGL((k_band<kEncode, false, FT>), grid, dim3(64), 0, stream, frames, states,
-                       tiles, tile_total, folded, (char*)folded, sink, queues + 1);
+    hipLaunchKernelGGL((k_band<kEncode, false, FT>), grid, dim3(64), lds, stream, frames, states,
+                       tiles, group_first, group_total, folded, ring_groups, queues + 1);
   }
 }
 
-template void launch_band<true, uint16_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                          uint16_t*, uint8_t*, int32_t*, bool, bool,
-                                          hipStream_t);
-template void launch_band<true, uint32_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                          uint32_t*, uint8_t*, int32_t*, bool, bool,
-                                          hipStream_t);
-template void launch_band<false, uint16_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                           uint16_t*, uint8_t*, int32_t*, bool, bool,
-                                           hipStream_t);
-template void launch_band<false, uint32_t>(const FrameDesc*, FrameState*, const TileDesc*, int,
-                                           uint32_t*, uint8_t*, int32_t*, bool, bool,
-                                           hipStream_t);
+#define FVH_BAND_INST(E, FT)                                                                  \
+  template void launch_band<E, FT>(const FrameDesc*, FrameState*, const TileDesc*,            \
+                                   const int32_t*, int, int, FT*, int32_t*, bool, bool,        \
+                                   hipStream_t);
+FVH_BAND_INST(true, uint16_t)
+FVH_BAND_INST(true, uint32_t)
+FVH_BAND_INST(false, uint16_t)
+FVH_BAND_INST(false, uint32_t)
+#undef FVH_BAND_INST
 
 }  // namespace fvh

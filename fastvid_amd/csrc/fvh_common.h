@@ -102,6 +102,8 @@ struct FrameDesc {
   int32_t step;                // decode: step from the header
   int32_t header_flags;        // decode: header byte 7 flag bits
   int32_t band;                // tiles eligible for the band wavefront (k_band)
+  int32_t group_begin[4];      // global index of each plane's first band tile group (groups
+                               // of 4 consecutive tiles of one plane); [3] = end of frame
 };
 
 // Per-frame state written on the device during a call.

@@ -849,7 +849,7 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
     hipLaunchKernelGGL(k_dstate_init, dim3((max(b.frame_count, kQueueCount) + 63) / 64), dim3(64),
                        0, stream, b.states, b.frame_count, b.list_count, b.queues);
   }
-  launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.tile_total, stream);
+  launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.group_first, b.tile_total, stream);
   if (b.fed_sizes != nullptr) {
     FVH_TIMED("k_dheader", stream);
     hipLaunchKernelGGL(k_dheader, fgrid, dim3(64), 0, stream, b.frames, b.states, b.frame_count,

@@ -1258,8 +1258,8 @@ template <typename FT>
 static void encode_pass(const EncodeBuffers& b, int pass, hipStream_t stream) {
   FT* folded = (FT*)b.folded;
   if (b.any_lossy && b.any_band) {
-    launch_band<true, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.sink,
-                          b.queues + 2 * pass, b.any_med, b.any_grad, stream);
+    launch_band<true, FT>(b.frames, b.states, b.tiles, b.group_first, b.group_total, b.band_ring,
+                          folded, b.queues + 2 * pass, b.any_med, b.any_grad, stream);
   }
   if (b.any_lossy && b.any_nonband) {
     {
@@ -1364,7 +1364,7 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
 }
 
 void launch_encode(const EncodeBuffers& b, hipStream_t stream) {
-  launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.tile_total, stream);
+  launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.group_first, b.tile_total, stream);
   if (b.wide_folded) encode_all<uint32_t>(b, stream);
   else encode_all<uint16_t>(b, stream);
 }
@@ -1374,7 +1374,8 @@ template <typename FT>
 void launch_reconstruct(const DecodeBuffers& b, hipStream_t stream) {
   FT* folded = (FT*)b.folded;
   if (b.any_band) {
-    launch_band<false, FT>(b.frames, b.states, b.tiles, b.tile_total, folded, b.sink, b.queues,
+    launch_band<false, FT>(b.frames, b.states, b.tiles, b.group_first, b.group_total,
+                           b.band_ring, folded, b.queues,
                            b.any_med, b.any_grad, stream);
   }
   if (b.any_nonband) {

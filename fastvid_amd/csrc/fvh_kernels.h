@@ -24,6 +24,9 @@ struct EncodeBuffers {
   uint16_t* recon;       // tile-order reconstruction for tiles wider than kRingMax (or null)
   uint8_t* sink;         // kSinkBytes target of masked-off stores
   int32_t* queues;       // kQueueCount work counters, zeroed by k_state_init
+  int32_t* group_first;  // first tile of each band tile group (written by k_plan)
+  int group_total;
+  int band_ring;         // ring slots per tile of the band wavefront (widest band tile)
   Results* results;
   int frame_count;
   int tile_total;
@@ -51,6 +54,9 @@ struct DecodeBuffers {
   O0Tab* o0tabs;         // parsed order-0 tables, by order-0 list position
   uint8_t* sink;         // kSinkBytes target of masked-off stores
   int32_t* queues;       // kQueueCount work counters, zeroed by k_dstate_init
+  int32_t* group_first;
+  int group_total;
+  int band_ring;
   const int64_t* fed_sizes;  // device stream sizes (headers then parsed on the device), or null
   void* folded;
   Results* results;
@@ -70,7 +76,7 @@ struct DecodeBuffers {
 int device_cus();
 
 void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,
-                 int tile_total, hipStream_t stream);
+                 int32_t* group_first, int tile_total, hipStream_t stream);
 void launch_upload_frames(const FrameDesc* host_frames, int frame_count, FrameDesc* device_frames,
                           hipStream_t stream);
 void launch_encode(const EncodeBuffers& b, hipStream_t stream);
@@ -78,8 +84,8 @@ void launch_decode(const DecodeBuffers& b, hipStream_t stream);
 template <typename FT>
 void launch_reconstruct(const DecodeBuffers& b, hipStream_t stream);
 template <bool kEncode, typename FT>
-void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* tiles, int tile_total,
-                 FT* folded, uint8_t* sink, int32_t* queues, bool med, bool plain,
-                 hipStream_t stream);
+void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* tiles,
+                 const int32_t* group_first, int group_total, int ring_groups, FT* folded,
+                 int32_t* queues, bool med, bool plain, hipStream_t stream);
 
 }  // namespace fvh

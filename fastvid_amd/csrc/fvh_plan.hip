@@ -15,7 +15,8 @@ namespace fvh {
 
 __global__ __launch_bounds__(256) void k_plan(const FrameDesc* __restrict__ frames,
                                               int frame_count, TileDesc* __restrict__ tiles,
-                                              ShardDesc* __restrict__ shards, int tile_total) {
+                                              ShardDesc* __restrict__ shards,
+                                              int32_t* __restrict__ group_first, int tile_total) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= tile_total) return;
   int lo = 0, hi = frame_count - 1;
@@ -53,6 +54,8 @@ __global__ __launch_bounds__(256) void k_plan(const FrameDesc* __restrict__ fram
   td.first_shard = (int32_t)first;
   td.shard_count = (int32_t)count;
   tiles[t] = td;
+  // band tile groups: 4 consecutive tiles of one plane
+  if ((i & 3) == 0) group_first[f.group_begin[plane] + (i >> 2)] = t;
   for (int64_t s = 0; s < count; ++s) {
     ShardDesc sd;
     sd.tile = t;
@@ -77,12 +80,12 @@ int device_cus() {
 }
 
 void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,
-                 int tile_total, hipStream_t stream) {
+                 int32_t* group_first, int tile_total, hipStream_t stream) {
   if (tile_total <= 0) return;
   {
     FVH_TIMED("k_plan", stream);
     hipLaunchKernelGGL(k_plan, dim3((tile_total + 255) / 256), dim3(256), 0, stream, frames,
-                       frame_count, tiles, shards, tile_total);
+                       frame_count, tiles, shards, group_first, tile_total);
   }
 }
 

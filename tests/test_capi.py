@@ -173,9 +173,12 @@ def _reference_quantize(r, d):
 
 @pytest.mark.parametrize("bit_depth", (8, 10, 12))
 def test_band_quantiser_signed_offset_is_exact(bit_depth):
-    """The band wavefront's q = floor((r + off) * m >> s) - K, off = h + K*d
-    (r >= 0) or d - 1 - h + K*d (r < 0), s = bd + 2 + ceil(log2 d), equals
-    sign(r) * floor((|r| + d/2) / d) for every residual and reachable step."""
+    """The band wavefront's quotient: n = r + h + K*d - [r < 0 and d even],
+    s = bd + 2 + ceil(log2 d), m = ceil(2^s / d), then one 24-bit
+    multiply-add n*m + (2^32 - K*2^s) wrapped to 32 bits, read as int32 and
+    shifted arithmetically by s.  It equals sign(r) * floor((|r| + d/2) / d)
+    for every residual of every reachable step (a wrong even-step correction,
+    K or shift fails here)."""
     maxv = (1 << bit_depth) - 1
     r = np.arange(-maxv, maxv + 1, dtype=np.int64)
     steps = {1 + ((100 - q) * (1 << (bit_depth - 8)) + den - 1) // den
@@ -186,10 +189,16 @@ def test_band_quantiser_signed_offset_is_exact(bit_depth):
         c = (d - 1).bit_length()
         s = bit_depth + 2 + c
         m = -(-(1 << s) // d)
-        n = r + np.where(r < 0, d - 1 - h + K * d, h + K * d)
-        assert n.min() >= 0 and int(n.max()) * m < (1 << 32) and m < (1 << 24)
-        q = ((n * m) >> s) - K
+        offp = h + K * d
+        n = r + offp - ((r < 0) & (d % 2 == 0))
+        assert n.min() >= 0 and n.max() < (1 << 24) and m < (1 << 24)
+        negks = ((1 << 32) - (K << s)) % (1 << 32)
+        wrapped = (n * m + negks) % (1 << 32)
+        signed = np.where(wrapped >= (1 << 31), wrapped - (1 << 32), wrapped)
+        q = signed >> s
         assert np.array_equal(q, _reference_quantize(r, d)), d
+        # the even-step correction is exactly "numerator below h + K*d"
+        assert np.array_equal(r < 0, (r + offp) < offp)
 
 
 def test_band_quantiser_16bit_is_exact():
