@@ -44,6 +44,7 @@ constexpr int kBandRows = 16;
 constexpr int kRowChunks = 9;              // line buffer row: 8 chunks + 1 against bank conflicts
 constexpr uint32_t kOob = 0x80000000u;     // buffer offset past every resource: access dropped
 constexpr int kRsrcWord3 = 0x00020000;     // raw buffer, 32-bit dwords (gfx9 family)
+constexpr int kAuxNt = 2;                  // buffer instruction cache policy: non-temporal
 
 using v4u = uint32_t __attribute__((ext_vector_type(4)));
 using i16x2 = short __attribute__((ext_vector_type(2)));
@@ -82,8 +83,9 @@ struct Vec8<uint16_t> {
     return (j & 1) ? (w >> 16) : (w & 0xffffu);
   }
   __device__ __forceinline__ uint32_t orall() const { return v.x | v.y | v.z | v.w; }
+  // input lines are read once: non-temporal (aux bit 1, "nt")
   __device__ __forceinline__ void load(rsrc_t r, uint32_t off) {
-    v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kAuxNt);
   }
   __device__ __forceinline__ void store(rsrc_t r, uint32_t off) const {
     __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0);
@@ -109,8 +111,8 @@ struct Vec8<uint32_t> {
     }
   }
   __device__ __forceinline__ void load(rsrc_t r, uint32_t off) {
-    a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-    b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, 0);
+    a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kAuxNt);
+    b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, kAuxNt);
   }
   __device__ __forceinline__ void store(rsrc_t r, uint32_t off) const {
     __builtin_amdgcn_raw_buffer_store_b128(a, r, off, 0, 0);
@@ -298,9 +300,18 @@ __device__ __forceinline__ void band_group(
   // ---- line trackers: local time of the line as band bb and column group cg
   int ld_cg = -8 * half, ld_bb = 0;        // lines loaded in the iteration before `it`
   int st_cg = -8 * (1 + half), st_bb = 0;  // lines finishing in groups 0..6 of `it`
+  // lines that do not exist (before the row starts, past the last band or
+  // the tile's width) load from an offset the hardware drops: no traffic
+  int ld_left = 0;
   auto ld_base = [&]() __attribute__((always_inline)) {
-    return in_off + __umul24((uint32_t)(16 * ld_bb + 8 * half), in_pitch) +
-           (uint32_t)(ld_cg + c) * kCi;
+    const bool ok = ld_cg >= 0 && ld_bb < nb && ld_cg + c < WG;
+    ld_left = H - 16 * ld_bb - 8 * half;
+    return ok ? in_off + __umul24((uint32_t)(16 * ld_bb + 8 * half), in_pitch) +
+                    (uint32_t)(ld_cg + c) * kCi
+              : kOob;
+  };
+  auto ld_off = [&](uint32_t b, int k) __attribute__((always_inline)) {
+    return k < ld_left ? b + __umul24((uint32_t)k, in_pitch) : kOob;
   };
   uint32_t st_base = kOob;
   int st_left = 0;
@@ -330,14 +341,14 @@ __device__ __forceinline__ void band_group(
       slot.load(in_r, off);
       __builtin_amdgcn_sched_barrier(0);
     };
-    prime(s0, b, 0);
-    prime(s1, b + in_pitch, 1);
-    prime(s2, b + __umul24(2u, in_pitch), 2);
-    prime(s3, b + __umul24(3u, in_pitch), 3);
-    prime(s4, b + __umul24(4u, in_pitch), 4);
-    prime(s5, b + __umul24(5u, in_pitch), 5);
-    prime(s6, b + __umul24(6u, in_pitch), 6);
-    prime(s7, b + __umul24(7u, in_pitch), 7);
+    prime(s0, ld_off(b, 0), 0);
+    prime(s1, ld_off(b, 1), 1);
+    prime(s2, ld_off(b, 2), 2);
+    prime(s3, ld_off(b, 3), 3);
+    prime(s4, ld_off(b, 4), 4);
+    prime(s5, ld_off(b, 5), 5);
+    prime(s6, ld_off(b, 6), 6);
+    prime(s7, ld_off(b, 7), 7);
   }
 
   // ---- own lane state: cg = column group (negative before the row starts),
@@ -456,7 +467,7 @@ __device__ __forceinline__ void band_group(
     // (5) the line starting in the next iteration for rows k and k + 8, into
     // the slot just emptied (pinned in slot order, see above)
     __builtin_amdgcn_sched_barrier(0);
-    slot.load(in_r, ldb + __umul24((uint32_t)k, in_pitch));
+    slot.load(in_r, ld_off(ldb, k));
     __builtin_amdgcn_sched_barrier(0);
     // advance
     ++cg;
