@@ -23,13 +23,15 @@ __device__ __forceinline__ uint64_t rd_u64(const uint8_t* p) {
   return (uint64_t)rd_u32(p) | ((uint64_t)rd_u32(p + 4) << 32);
 }
 
+// Order-0 alphabet tiers of the decoder (symbols per shard, at most): each
+// tier has its own shard list and decoder instance.
+constexpr int kO0Tiers = 3;
+__host__ __device__ constexpr int o0_tier_cap(int t) { return t == 0 ? 16 : t == 1 ? 64 : 256; }
+
 __global__ void k_dstate_init(FrameState* __restrict__ states, int frame_count,
                               int32_t* __restrict__ list_count, int32_t* __restrict__ queues) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    list_count[0] = 0;
-    list_count[1] = 0;
-  }
+  if (i < 2 + kO0Tiers) list_count[i] = 0;  // order-0, other, one per order-0 tier
   if (i < kQueueCount) queues[i] = 0;
   if (i >= frame_count) return;
   FrameState s = {};
@@ -522,7 +524,8 @@ __global__ __launch_bounds__(256) void k_o0_parse(const FrameDesc* __restrict__ 
                                                   FrameState* __restrict__ states,
                                                   const DShard* __restrict__ dshards,
                                                   const int32_t* __restrict__ lists,
-                                                  const int32_t* __restrict__ list_count,
+                                                  int32_t* __restrict__ list_count,
+                                                  int32_t* __restrict__ tier_lists, int tier_stride,
                                                   O0Tab* __restrict__ tabs, char* gbase) {
   const int item = blockIdx.x * blockDim.x + threadIdx.x;
   if (item >= list_count[0]) return;
@@ -637,23 +640,55 @@ __global__ __launch_bounds__(256) void k_o0_parse(const FrameDesc* __restrict__ 
   }
   tab->ent[nsym] = 1u << kRansTableLog;
   tab->nsym = (int32_t)nsym;
+  // file the shard for the decoder instance sized for its alphabet: one
+  // atomic per wave and tier (every valid shard of the wave is here)
+  const int tier = nsym <= (uint32_t)o0_tier_cap(0) ? 0 : nsym <= (uint32_t)o0_tier_cap(1) ? 1 : 2;
+  const int lane = lane_id();
+#pragma unroll
+  for (int t = 0; t < kO0Tiers; ++t) {
+    const uint64_t m = ballot(tier == t);
+    if (m == 0) continue;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&list_count[2 + t], popc64(m));
+    base = shfl(base, leader);
+    if (tier == t) {
+      tier_lists[t * tier_stride + base + popc64(m & ((1ull << lane) - 1))] = item;
+    }
+  }
 }
 
 // Order-0 rANS lanes (decode.cu:511-549): sixteen shards per wave, one quad
-// of lanes per shard, lane r of the quad running rANS lane r.  The slot ->
-// rank map is a 256-entry bucket table in LDS: bucket b covers slots
-// [16 b, 16 b + 16) and holds the rank at its first slot (bits 0-7) and a mask
-// of the offsets 1..15 where a new rank starts (bits 17-31), so the rank of a
-// slot is the first rank plus a popcount.  Bucket and entry table take 2 KB
-// per shard.
+// of lanes per shard, lane r of the quad running rANS lane r.  Waves are
+// persistent and take groups of 16 shards from one of three tier lists, by
+// alphabet size (at most 16, 64 or 255 symbols; k_o0_parse files each
+// shard), so the per-shard tables in LDS are sized by the alphabet and small
+// alphabets leave room for more waves on a compute unit.
+//
+// Slot -> rank: the 4096 slots form 128 buckets of 32; mask[b] has bit j set
+// where a symbol's slot range starts at slot 32 b + j (the first symbol
+// excluded) and prefix[b] counts the starts before the bucket, so the rank of
+// a slot is prefix[b] + popcount(mask[b] & ((2 << j) - 1)).  Rank -> symbol:
+// ent[k] = cum_k | (freq_k - 4096 mod 2^13) << 13 and, at a fixed distance
+// in the same row, the value, read together by one two-dword LDS read.  The
+// state update freq*(x >> 12) + (x & 4095) - cum is computed as
+// (freq - 4096)*(x >> 12) + (x - cum), one signed 24-bit multiply-add whose
+// low 32 bits equal the reference's (exact) u32 result.  Tables: 640 bytes +
+// 8 per symbol, against the reference's 4096-byte slot table
+// (decode.cu:519-527).
 //
 // The symbol loop runs in rounds of four steps with uniform control flow: a
-// step renormalises with at most two bytes, so a round needs at most eight,
-// taken big-endian from a two-chunk window of the lane's bytes; the chunk
-// after the window is loaded one round ahead, unconditionally, and steps past
-// the lane's end store to a sink slot.  Every round thus issues the same
-// memory operations, and the only wait on memory is for that chunk.
+// step renormalises with at most two bytes (a decoded state is >= 2^11), so a
+// round needs at most eight, taken big-endian from a window of three chunks
+// of the lane's bytes (see `iteration`).  While every lane of the wave has
+// four symbols left the rounds run unpredicated; a tail loop finishes lanes of
+// uneven length.  Decoded values go to a
+// 32-symbol stage per shard in LDS and leave as one 16-byte store per lane
+// every two rounds.  Errors (decode.cu:536-548): a lane that needs a byte past
+// its end consumes more bytes than it has, so the final check "bytes consumed
+// == lane length and state == 2^23" covers every failure of the reference.
 constexpr int kO0Shards = 16;
+constexpr int kO0Buckets = 128;
 
 // A 16-byte chunk as two little-endian quadwords.  (Chunks are kept as
 // scalars: a conditional between two vector-typed objects would be compiled
@@ -678,125 +713,234 @@ __device__ __forceinline__ uint64_t window8_be(uint64_t a0, uint64_t a1, uint64_
   return __builtin_bswap64(le);
 }
 
+// popcount(a) + b in one instruction (the compiler otherwise distributes the
+// later scaling of the sum over both terms)
+__device__ __forceinline__ uint32_t bcnt_add(uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// Entry rows: kMax entries, then the values at a distance the two-dword LDS
+// read can express (up to 255 dwords, or a multiple of 64).  Row and stage
+// strides are odd in dwords: the sixteen shards of a wave mostly read the same
+// few ranks, and an even stride would put them on a few banks.
+template <int kMax>
+struct O0Row {
+  static constexpr int kEnt = (kMax + 3) & ~3;
+  static constexpr int kVal = kEnt <= 128 ? kEnt : 256;
+  static constexpr int kWords = (kVal + kEnt) | 1;
+};
+
 template <typename FT>
+struct O0Stage {
+  static constexpr int kWords = (32 * (int)sizeof(FT) / 4) | 1;
+};
+
+template <typename FT, int kMax>
+struct O0Lds {
+  uint32_t mask[kO0Shards][kO0Buckets];
+  uint32_t row[kO0Shards][O0Row<kMax>::kWords];
+  uint8_t prefix[kO0Shards][kO0Buckets];
+  uint32_t stage[kO0Shards][O0Stage<FT>::kWords];
+};
+
+template <typename FT, int kMax>
 __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ frames,
                                                const DShard* __restrict__ dshards,
                                                const int32_t* __restrict__ lists,
-                                               const int32_t* __restrict__ list_count,
+                                               const int32_t* __restrict__ tier_list,
+                                               const int32_t* __restrict__ tier_count,
                                                O0Tab* __restrict__ tabs,
                                                FrameState* __restrict__ states,
-                                               FT* __restrict__ folded, char* gbase) {
-  __shared__ uint32_t bkt_all[kO0Shards][256];
-  __shared__ uint32_t ent_all[kO0Shards][256];
+                                               FT* __restrict__ folded, char* gbase,
+                                               uint8_t* __restrict__ sink,
+                                               int32_t* __restrict__ queue) {
+  using Row = O0Row<kMax>;
+  __shared__ O0Lds<FT, kMax> lds;
   const int lane = lane_id();
   const int g = lane >> 2;
   const int r = lane & 3;
-  const int item = blockIdx.x * kO0Shards + g;
-  uint32_t* bkt = bkt_all[g];
-  uint32_t* ent = ent_all[g];
-  O0Tab* tab = tabs + item;
-  const int nsym = item < list_count[0] ? tab->nsym : 0;
-  if (nsym > 0) {
-    const uint4* src = (const uint4*)tab->ent;
-    for (int j = r; 4 * j <= nsym; j += 4) ((uint4*)ent)[j] = src[j];
-  }
-  __syncthreads();
-  if (nsym > 0) {
-    // buckets [64 r, 64 r + 64): start from the rank holding slot 1024 r
-    auto cum = [&](int k) { return (int)(ent[k] & 0x1fffu); };
-    int lo = 0, hi = nsym;  // cum(lo) <= 1024 r < cum(hi)
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (cum(mid) <= 1024 * r) lo = mid;
-      else hi = mid;
-    }
-    int k = lo;
-    int next = cum(k + 1);
-    for (int bi = 0; bi < 64; ++bi) {
-      const int base = (64 * r + bi) * 16;
-      while (next <= base) {
-        ++k;
-        next = cum(k + 1);
-      }
-      uint32_t mask = 0;
-      int j = k + 1;
-      int cj = next;
-      while (cj < base + 16) {
-        mask |= 1u << (cj - base);
-        ++j;
-        cj = cum(j);
-      }
-      bkt[64 * r + bi] = (uint32_t)k | (mask << 16);
-    }
-  }
-  __syncthreads();
-  if (nsym == 0) return;
-  const DShard d = dshards[lists[item]];
-  const FrameDesc& f = frames[d.frame];
-  const uint8_t* body = as_global(gbase, (const uint8_t*)f.stream + d.body_offset);
-  const int length = tab->length[r];
-  uint32_t x = tab->state[r];
-  // aligned chunks holding the lane's bytes (a lane without bytes reads the
-  // body's first chunk, and never uses it)
-  const uint8_t* first = length > 0 ? body + tab->start[r] : body;
-  const int head = (int)((uintptr_t)first & 15);
-  const uint4* cp = (const uint4*)(first - head);
-  const int last = length > 0 ? (head + length - 1) >> 4 : 0;
-  const Chunk A0 = load_chunk(cp);
-  const Chunk B0 = load_chunk(cp + min(1, last));
-  const Chunk C0 = load_chunk(cp + min(2, last));
-  uint64_t a0 = A0.lo, a1 = A0.hi, b0 = B0.lo, b1 = B0.hi, c0 = C0.lo, c1 = C0.hi;
-  int ci = 0;       // chunk index of A
-  int pos = head;   // next byte within A:B, < 16 at the start of a round
-  int rem = length; // lane bytes not yet consumed
-  FT* out = folded + d.folded_offset + r;
-  FT* sink = (FT*)&tab->sink[r];
-  const int n = d.count;
-  const int nsteps = n > r ? (n - r + 3) >> 2 : 0;
-  int bad = 0;
-  // settle the window loads here: otherwise the wait for them stays at the
-  // top of the loop, where it would also wait for every earlier store
-  __builtin_amdgcn_s_waitcnt(0);
-  for (int s0 = 0; s0 < nsteps; s0 += 4) {
-    uint64_t bb = window8_be(a0, a1, b0, pos);
-    int used = 0;
+  const int count = *tier_count;
+  uint32_t* const mask = lds.mask[g];
+  uint32_t* const ent = lds.row[g];
+  uint8_t* const prefix = lds.prefix[g];
+  FT* const stage = (FT*)lds.stage[g];
+  for (;;) {
+    int grp = 0;
+    if (lane == 0) grp = atomicAdd(queue, 1);
+    grp = __builtin_amdgcn_readfirstlane(grp);
+    if (grp * kO0Shards >= count) break;
+    const int at = grp * kO0Shards + g;
+    const int item = at < count ? tier_list[at] : -1;
+    const O0Tab* tab = tabs + (item >= 0 ? item : 0);
+    const int nsym = item >= 0 ? tab->nsym : 0;
+    // ---- tables: entries and values, boundary masks, prefix counts
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const bool live = s0 + t < nsteps;
-      const uint32_t slot = x & ((1u << kRansTableLog) - 1);
-      const uint32_t e = bkt[slot >> 4];
-      const int rank = (int)(e & 0xffu) + __popc((e >> 16) & ((2u << (slot & 15u)) - 2u));
-      const uint32_t lo_e = ent[rank];
-      const uint32_t hi_e = ent[rank + 1];
-      const uint32_t c = lo_e & 0x1fffu;
-      const uint32_t fr = (hi_e & 0x1fffu) - c;
-      FT* dst = live ? out + 4 * (s0 + t) : sink;
-      *dst = (FT)(lo_e >> 13);
-      const uint32_t xd = __umul24(fr, x >> kRansTableLog) + slot - c;
-      // one byte below 2^23, two below 2^15
-      const int nb = live ? (int)(xd < kRansL) + (int)(xd < (1u << 15)) : 0;
-      const uint32_t top = (uint32_t)(bb >> 48);
-      const uint32_t x1 = (xd << 8) | (top >> 8);
-      const uint32_t x2 = (xd << 16) | top;
-      x = !live ? x : nb == 0 ? xd : nb == 1 ? x1 : x2;
-      bb <<= 8 * nb;
-      used += nb;
+    for (int j = 0; j < 8; ++j) ((uint4*)mask)[8 * r + j] = make_uint4(0, 0, 0, 0);
+    __builtin_amdgcn_wave_barrier();
+    for (int k = r; k < nsym; k += 4) {
+      const uint32_t e0 = tab->ent[k];
+      const uint32_t c = e0 & 0x1fffu;
+      const uint32_t fr = (tab->ent[k + 1] & 0x1fffu) - c;
+      ent[k] = c | (((fr - 4096u) & 0x1fffu) << 13);
+      ent[Row::kVal + k] = e0 >> 13;
+      if (k > 0) atomicOr(&mask[c >> 5], 1u << (c & 31u));
     }
-    bad |= used > min(rem, 8);
-    rem -= used;
-    pos += used;
-    const bool adv = pos >= 16;
-    a0 = adv ? b0 : a0;
-    a1 = adv ? b1 : a1;
-    b0 = adv ? c0 : b0;
-    b1 = adv ? c1 : b1;
-    pos -= adv ? 16 : 0;
-    ci += adv ? 1 : 0;
-    const Chunk Cn = load_chunk(cp + min(ci + 2, last));
-    c0 = Cn.lo;
-    c1 = Cn.hi;
+    __builtin_amdgcn_wave_barrier();
+    {
+      // lane r: buckets 32 r .. 32 r + 31
+      uint32_t cnt[32];
+      uint32_t total = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint4 w = ((const uint4*)mask)[8 * r + j];
+        cnt[4 * j + 0] = total;
+        total += __popc(w.x);
+        cnt[4 * j + 1] = total;
+        total += __popc(w.y);
+        cnt[4 * j + 2] = total;
+        total += __popc(w.z);
+        cnt[4 * j + 3] = total;
+        total += __popc(w.w);
+      }
+      // exclusive scan of the four lane totals of the quad
+      const uint32_t t1 = __shfl_up(total, 1, 4);
+      const uint32_t s1 = total + (r >= 1 ? t1 : 0u);
+      const uint32_t t2 = __shfl_up(s1, 2, 4);
+      const uint32_t incl = s1 + (r >= 2 ? t2 : 0u);
+      const uint32_t base = incl - total;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t w = (base + cnt[4 * j]) | ((base + cnt[4 * j + 1]) << 8) |
+                           ((base + cnt[4 * j + 2]) << 16) | ((base + cnt[4 * j + 3]) << 24);
+        ((uint32_t*)prefix)[8 * r + j] = w;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // ---- lane state
+    DShard d = {};
+    if (item >= 0) d = dshards[lists[item]];
+    const FrameDesc& f = frames[d.frame];
+    const uint8_t* body = as_global(gbase, (const uint8_t*)f.stream + d.body_offset);
+    const int length = nsym > 0 ? tab->length[r] : 0;
+    uint32_t x = nsym > 0 ? tab->state[r] : kRansL;
+    // aligned chunks holding the lane's bytes (a lane without bytes reads the
+    // body's first chunk, and never uses it)
+    const uint8_t* first = length > 0 ? body + tab->start[r] : body;
+    const int head = (int)((uintptr_t)first & 15);
+    const uint4* cp = (const uint4*)(first - head);
+    const int last = length > 0 ? (head + length - 1) >> 4 : 0;
+    const Chunk A0 = load_chunk(cp);
+    const Chunk B0 = load_chunk(cp + min(1, last));
+    const Chunk C0 = load_chunk(cp + min(2, last));
+    uint64_t a0 = A0.lo, a1 = A0.hi, b0 = B0.lo, b1 = B0.hi, c0 = C0.lo, c1 = C0.hi;
+    int ci = 0;          // chunk index of A
+    int pos = head;      // next byte within A:B, < 16 when an iteration starts
+    int consumed = 0;    // lane bits consumed
+    const int n = nsym > 0 ? d.count : 0;
+    FT* const out = folded + d.folded_offset;
+    const int nsteps = n > r ? (n - r + 3) >> 2 : 0;
+    // shards without work (past the list's end, rejected tables) do not hold
+    // back the unpredicated rounds; their lanes decode nothing of use and
+    // store to a sink slot
+    const int full_steps = __builtin_amdgcn_readfirstlane(
+        (int)wave_min((uint32_t)(nsym > 0 ? nsteps : 0x7fffffff)));
+    const int wave_steps = __builtin_amdgcn_readfirstlane((int)wave_max((uint32_t)nsteps));
+    FT* const flush_base = nsym > 0 ? out : (FT*)(sink + lane * kLaneSinkBytes);
+    const int flush_step = nsym > 0 ? 1 : 0;
+    // settle the window loads here: otherwise the wait for them stays at the
+    // top of the loop, where it would also wait for every earlier store
+    __builtin_amdgcn_s_waitcnt(0);
+
+    // one step: slot -> rank -> state update and renormalisation
+    auto step = [&](uint64_t& bb, int& used, int slot_pos, bool live) __attribute__((always_inline)) {
+      const uint32_t b = (x >> 5) & (kO0Buckets - 1);
+      const uint32_t m = mask[b] & ((2u << (x & 31u)) - 1u);
+      const uint32_t rank = bcnt_add(m, prefix[b]);
+      const uint32_t e = ent[rank];
+      const uint32_t v = ent[Row::kVal + rank];
+      stage[4 * slot_pos + r] = (FT)v;
+      const uint32_t c = e & 0x1fffu;
+      const int fneg = ((int)(e << 6)) >> 19;  // freq - 4096, sign-extended
+      const uint32_t xd = (uint32_t)__mul24(fneg, (int)(x >> kRansTableLog)) + (x - c);
+      // one byte below 2^23, two below 2^15 (never more: xd >= 2^11)
+      int sh = (xd < kRansL ? 8 : 0) + (xd < (1u << 15) ? 8 : 0);
+      if (!live) sh = 0;
+      const uint64_t wide = (((uint64_t)xd << 32) | (uint32_t)(bb >> 32)) << sh;
+      x = live ? (uint32_t)(wide >> 32) : x;
+      bb <<= sh;
+      used += sh;
+    };
+    // the 32 symbols of two rounds: lane r stores symbols 8 r .. 8 r + 7
+    auto flush_full = [&](int s0) __attribute__((always_inline)) {
+      FT* const dst = flush_base + (4 * s0 + 8 * r) * flush_step;
+      if constexpr (sizeof(FT) == 2) {
+        *(uint4*)dst = *(const uint4*)(stage + 8 * r);
+      } else {
+        ((uint4*)dst)[0] = ((const uint4*)(stage + 8 * r))[0];
+        ((uint4*)dst)[1] = ((const uint4*)(stage + 8 * r))[1];
+      }
+    };
+    auto flush = [&](int s0) __attribute__((always_inline)) {
+      const int p0 = 4 * s0 + 8 * r;
+      if (p0 + 8 <= n) {
+        flush_full(s0);
+      } else {
+        for (int j = 0; p0 + j < n; ++j) out[p0 + j] = stage[8 * r + j];
+      }
+    };
+    // Two rounds: the first reads A:B (pos < 16), the second A:B:C (pos < 24).
+    // After them the window moves by one chunk at most, and the chunk after
+    // the new window loads before the iteration's store, so waiting for it
+    // next iteration never waits for that store.
+    auto iteration = [&](int s0, bool predicated) __attribute__((always_inline)) {
+      int used_total = 0;
+      {
+        uint64_t bb = window8_be(a0, a1, b0, pos);
+        int used = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) step(bb, used, t, !predicated || s0 + t < nsteps);
+        used_total = used;
+      }
+      {
+        const int p1 = pos + (used_total >> 3);
+        const bool hi_half = p1 >= 16;
+        uint64_t bb = window8_be(hi_half ? b0 : a0, hi_half ? b1 : a1, hi_half ? c0 : b0, p1 & 15);
+        int used = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) step(bb, used, 4 + t, !predicated || s0 + 4 + t < nsteps);
+        used_total += used;
+      }
+      consumed += used_total;
+      pos += used_total >> 3;
+      const bool adv = pos >= 16;
+      a0 = adv ? b0 : a0;
+      a1 = adv ? b1 : a1;
+      b0 = adv ? c0 : b0;
+      b1 = adv ? c1 : b1;
+      pos -= adv ? 16 : 0;
+      ci += adv ? 1 : 0;
+      __builtin_amdgcn_sched_barrier(0);
+      const Chunk Cn = load_chunk(cp + min(ci + 2, last));
+      c0 = Cn.lo;
+      c1 = Cn.hi;
+      __builtin_amdgcn_sched_barrier(0);
+    };
+
+    int s0 = 0;
+    for (; s0 + 8 <= full_steps; s0 += 8) {
+      iteration(s0, false);
+      // every lane of the wave has all 32 symbols here (full_steps >= s0 + 8):
+      // one unconditional store, so the memory waits stay exact
+      flush_full(s0);
+    }
+    for (; s0 < wave_steps; s0 += 8) {
+      iteration(s0, true);
+      flush(s0);
+    }
+    if (nsym > 0 && (consumed != 8 * length || x != kRansL)) set_status(&states[d.frame], 5);
   }
-  if (bad || rem != 0 || x != kRansL) set_status(&states[d.frame], 5);
 }
 
 // predictor="serial": one thread per tile in raster order (decode.cu:608-656).
@@ -864,14 +1008,24 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
   {
     FVH_TIMED("k_o0_parse", stream);
     hipLaunchKernelGGL(k_o0_parse, dim3((b.shard_total + 255) / 256), dim3(256), 0, stream,
-                       b.frames, b.states, b.dshards, b.lists, b.list_count, b.o0tabs,
-                       (char*)b.folded);
+                       b.frames, b.states, b.dshards, b.lists, b.list_count, b.lists + 2 * b.shard_total,
+                       b.shard_total, b.o0tabs, (char*)b.folded);
   }
   {
     FVH_TIMED("k_dec_o0", stream);
-    hipLaunchKernelGGL((k_dec_o0<FT>), dim3((b.shard_total + kO0Shards - 1) / kO0Shards),
-                       dim3(64), 0, stream, b.frames, b.dshards, b.lists, b.list_count, b.o0tabs,
-                       b.states, (FT*)b.folded, (char*)b.folded);
+    // one persistent instance per alphabet tier; each drains its list
+    const int groups = (b.shard_total + kO0Shards - 1) / kO0Shards;
+    const dim3 grid(min(groups, 16 * device_cus()));
+    int32_t* tiers = b.lists + 2 * b.shard_total;
+    hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(0)>), grid, dim3(64), 0, stream, b.frames,
+                       b.dshards, b.lists, tiers, b.list_count + 2, b.o0tabs, b.states,
+                       (FT*)b.folded, (char*)b.folded, b.sink, b.queues + 4);
+    hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(1)>), grid, dim3(64), 0, stream, b.frames,
+                       b.dshards, b.lists, tiers + b.shard_total, b.list_count + 3, b.o0tabs,
+                       b.states, (FT*)b.folded, (char*)b.folded, b.sink, b.queues + 5);
+    hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(2) - 1>), grid, dim3(64), 0, stream, b.frames,
+                       b.dshards, b.lists, tiers + 2 * b.shard_total, b.list_count + 4, b.o0tabs,
+                       b.states, (FT*)b.folded, (char*)b.folded, b.sink, b.queues + 6);
   }
   {
     FVH_TIMED("k_dec_other", stream);

@@ -28,6 +28,12 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
   return v;
 }
 
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = min(v, (uint32_t)shfl_xor(v, m));
+  return v;
+}
+
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) v = max(v, (uint32_t)shfl_xor(v, m));

@@ -7,8 +7,8 @@ namespace fvh {
 
 constexpr int kRingMax = 4096;     // widest tile handled by the wave-per-tile wavefront
 constexpr int kBandRingMax = 512;  // widest tile handled by the band wavefront
-constexpr int kBandSinkBytes = 32; // per-lane sink slot of the band wavefront (8 x uint32)
-constexpr int kSinkBytes = 256 * kBandSinkBytes;  // workspace sink region
+constexpr int kLaneSinkBytes = 32; // per-lane sink slot for masked-off stores (8 x uint32)
+constexpr int kSinkBytes = 256 * kLaneSinkBytes;  // workspace sink region
 constexpr int kQueueCount = 8;     // work counters per call (two per predictor pass)
 
 struct EncodeBuffers {
@@ -49,8 +49,9 @@ struct DecodeBuffers {
   TileDesc* tiles;
   ShardDesc* shards;
   DShard* dshards;
-  int32_t* lists;        // [0, shard_total): order-0 shards, [shard_total, 2x): others
-  int32_t* list_count;   // [0] order-0, [1] others
+  int32_t* lists;        // [0, n): order-0 shards, [n, 2n): others, [(2+t) n, (3+t) n):
+                         // order-0 list positions of alphabet tier t (n = shard_total)
+  int32_t* list_count;   // [0] order-0, [1] others, [2 + t] tier t
   O0Tab* o0tabs;         // parsed order-0 tables, by order-0 list position
   uint8_t* sink;         // kSinkBytes target of masked-off stores
   int32_t* queues;       // kQueueCount work counters, zeroed by k_dstate_init
