@@ -792,6 +792,12 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
 // Control flow is uniform across a round, so the loop waits on memory only
 // for the rank chunk.  The last step decides the shard's mode: order-0 when
 // forced or strictly smaller than the best of three (encode.cu:423-429).
+//
+// Table capacity: a wave takes groups of 16 consecutive shards, and one
+// instance of the kernel per capacity (16, 64, 256 entries) takes the groups
+// whose largest alphabet falls in its range (the others skip them), so the
+// LDS held per wave follows the alphabets actually present: 2 KB for small
+// alphabets, which lets a compute unit hold every wave its registers allow.
 constexpr int kRansShards = 16;
 
 // Eight bytes starting at byte `pos` (0..15) of the 32-byte window a0:a1:b0:b1,
@@ -805,21 +811,18 @@ __device__ __forceinline__ uint64_t window8_le(uint64_t a0, uint64_t a1, uint64_
 }
 
 template <typename FT>
-__global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frames,
-                                             FrameState* __restrict__ states,
-                                             const TileDesc* __restrict__ tiles,
-                                             const ShardDesc* __restrict__ shards,
-                                             int shard_total, const FT* __restrict__ folded,
-                                             ShardRec* __restrict__ recs,
-                                             uint8_t* __restrict__ scratch,
-                                             uint8_t* __restrict__ sink) {
-  (void)folded;
+__device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
+                                           FrameState* __restrict__ states,
+                                           const TileDesc* __restrict__ tiles,
+                                           const ShardDesc* __restrict__ shards, int shard_total,
+                                           ShardRec* __restrict__ recs,
+                                           uint8_t* __restrict__ scratch,
+                                           uint8_t* __restrict__ sink, uint2* ent, int grp,
+                                           int cap, int cap_below) {
   using q2 = uint64_t __attribute__((ext_vector_type(2)));
-  __shared__ uint2 ent_all[kRansShards][256];
   const int lane = lane_id();
-  const int g = lane >> 2;
   const int r = lane & 3;
-  const int s = blockIdx.x * kRansShards + g;
+  const int s = grp * kRansShards + (lane >> 2);
   bool act = false;
   int count = 0, distinct = 0;
   TileDesc td = {};
@@ -833,8 +836,11 @@ __global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frame
     count = (int)min((int64_t)kShardSymbols, (int64_t)td.width * td.height - local0);
     distinct = recs[s].distinct;
   }
+  // the group belongs to the instance whose capacity range holds its largest
+  // alphabet
+  const int largest = __builtin_amdgcn_readfirstlane((int)wave_max((uint32_t)(act ? distinct : 0)));
+  if (largest > cap || largest <= cap_below) return;
   uint8_t* sbase = scratch + (int64_t)(act ? s : 0) * kScratchBytes;
-  uint2* ent = ent_all[g];
   if (act) {
     const uint2* tab = (const uint2*)(sbase + kScratchTable);
     for (int k = r; k < distinct; k += 4) ent[k] = tab[k];
@@ -932,6 +938,24 @@ __global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frame
       rec.lane[2] = (uint16_t)b2;
       rec.lane[3] = (uint16_t)b3;
     }
+  }
+}
+
+template <typename FT, int kCap>
+__global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frames,
+                                             FrameState* __restrict__ states,
+                                             const TileDesc* __restrict__ tiles,
+                                             const ShardDesc* __restrict__ shards,
+                                             int shard_total, const FT* __restrict__ folded,
+                                             ShardRec* __restrict__ recs,
+                                             uint8_t* __restrict__ scratch,
+                                             uint8_t* __restrict__ sink, int cap_below) {
+  (void)folded;
+  __shared__ uint2 ent_all[kRansShards][kCap];
+  const int groups = (shard_total + kRansShards - 1) / kRansShards;
+  for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+    rans_group<FT>(frames, states, tiles, shards, shard_total, recs, scratch, sink,
+                   ent_all[lane_id() >> 2], grp, kCap, cap_below);
   }
 }
 
@@ -1290,9 +1314,15 @@ static void encode_pass(const EncodeBuffers& b, int pass, hipStream_t stream) {
   }
   {
     FVH_TIMED("k_rans", stream);
-    hipLaunchKernelGGL((k_rans<FT>), dim3((b.shard_total + kRansShards - 1) / kRansShards), dim3(64), 0, stream,
-                       b.frames, b.states, b.tiles, b.shards, b.shard_total, folded, b.recs,
-                       b.scratch, b.sink);
+    // one persistent instance per table capacity; each takes its groups
+    const int groups = (b.shard_total + kRansShards - 1) / kRansShards;
+    const dim3 grid(min(groups, 32 * device_cus()));
+    hipLaunchKernelGGL((k_rans<FT, 16>), grid, dim3(64), 0, stream, b.frames, b.states, b.tiles,
+                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.sink, -1);
+    hipLaunchKernelGGL((k_rans<FT, 64>), grid, dim3(64), 0, stream, b.frames, b.states, b.tiles,
+                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.sink, 16);
+    hipLaunchKernelGGL((k_rans<FT, 256>), grid, dim3(64), 0, stream, b.frames, b.states, b.tiles,
+                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.sink, 64);
   }
 }
 
