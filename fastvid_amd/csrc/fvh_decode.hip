@@ -844,9 +844,9 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
     // shards without work (past the list's end, rejected tables) do not hold
     // back the unpredicated rounds; their lanes decode nothing of use and
     // store to a sink slot
-    const int full_steps = __builtin_amdgcn_readfirstlane(
-        (int)wave_min((uint32_t)(nsym > 0 ? nsteps : 0x7fffffff)));
     const int wave_steps = __builtin_amdgcn_readfirstlane((int)wave_max((uint32_t)nsteps));
+    const int full_steps = min(wave_steps, __builtin_amdgcn_readfirstlane((int)wave_min(
+                                               (uint32_t)(nsym > 0 ? nsteps : 0x7fffffff))));
     FT* const flush_base = nsym > 0 ? out : (FT*)(sink + lane * kLaneSinkBytes);
     const int flush_step = nsym > 0 ? 1 : 0;
     // settle the window loads here: otherwise the wait for them stays at the

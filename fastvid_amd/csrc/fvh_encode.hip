@@ -782,21 +782,26 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
 
 // Order-0 rANS encode (encode.cu:398-418): one thread per rANS lane, four
 // consecutive threads per shard, 16 shards per wave (one wave per block).
-// The shard's rank-indexed entries live in LDS.  A lane reads its rank
-// region forwards in rounds of four symbols, taken from a two-chunk window
-// whose next chunk is loaded a round ahead; division by the symbol frequency
-// is the reciprocal multiply of rans_entry.  Renormalisation bytes collect in
-// a 128-bit register queue, and every round stores its oldest eight bytes
-// (or, with fewer pending, a sink slot) to the end of the lane's scratch
-// area, backwards, which thus ends with the lane's bytes in decode order.
-// Control flow is uniform across a round, so the loop waits on memory only
-// for the rank chunk.  The last step decides the shard's mode: order-0 when
-// forced or strictly smaller than the best of three (encode.cu:423-429).
+// The shard's rank-indexed entries live in LDS as {freq << 19, cum |
+// (4096 - freq) << 12, 1/freq rounded up to a double}: the renormalisation
+// test is two compares, and floor(x / freq) is trunc(double(x) * 1/freq),
+// exact for every state the encoder divides (x < freq * 2^19 <= 2^31).  A
+// lane reads its rank region forwards, 16 symbols per block from a two-chunk
+// window whose next chunk loads while the block runs.  A step's
+// renormalisation bytes go into the round's 64-bit word, a round (four
+// steps) merges them into a queue of fewer than eight bytes, and when eight
+// are queued it stores the oldest eight to the end of the lane's scratch
+// area, backwards, which thus ends with the lane's bytes in decode order;
+// rounds with nothing to store issue a store the hardware drops, so control
+// flow and memory waits are the same every round.  Rounds run unpredicated
+// while every active lane has 16 symbols left.  The last step decides the
+// shard's mode: order-0 when forced or strictly smaller than the best of
+// three (encode.cu:423-429).
 //
 // Table capacity: a wave takes groups of 16 consecutive shards, and one
 // instance of the kernel per capacity (16, 64, 256 entries) takes the groups
 // whose largest alphabet falls in its range (the others skip them), so the
-// LDS held per wave follows the alphabets actually present: 2 KB for small
+// LDS held per wave follows the alphabets actually present: 4 KB for small
 // alphabets, which lets a compute unit hold every wave its registers allow.
 constexpr int kRansShards = 16;
 
@@ -817,7 +822,7 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
                                            const ShardDesc* __restrict__ shards, int shard_total,
                                            ShardRec* __restrict__ recs,
                                            uint8_t* __restrict__ scratch,
-                                           uint8_t* __restrict__ sink, uint2* ent, int grp,
+                                           uint8_t* __restrict__ sink, uint4* ent, int grp,
                                            int cap, int cap_below) {
   using q2 = uint64_t __attribute__((ext_vector_type(2)));
   const int lane = lane_id();
@@ -842,101 +847,142 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   if (largest > cap || largest <= cap_below) return;
   uint8_t* sbase = scratch + (int64_t)(act ? s : 0) * kScratchBytes;
   if (act) {
+    // LDS entry of a rank: {freq << 19, cum | (4096 - freq) << 12, 1/freq
+    // rounded up to a double}; trunc(double(x) * that) == floor(x / freq) for
+    // every renormalised state (x < freq * 2^19 <= 2^31), freq 1 included
     const uint2* tab = (const uint2*)(sbase + kScratchTable);
-    for (int k = r; k < distinct; k += 4) ent[k] = tab[k];
+    for (int k = r; k < distinct; k += 4) {
+      const uint32_t w = tab[k].x;
+      const uint32_t freq = w & 0x1fffu;
+      const uint32_t cum = (w >> 13) & 0xfffu;
+      double rc = 1.0 / (double)freq;
+      if (__fma_rn(rc, (double)freq, -1.0) < 0.0) {
+        rc = __longlong_as_double(__double_as_longlong(rc) + 1);
+      }
+      const uint64_t rb = (uint64_t)__double_as_longlong(rc);
+      ent[k] = make_uint4(freq << 19, cum | ((4096u - freq) << 12), (uint32_t)rb,
+                          (uint32_t)(rb >> 32));
+    }
   }
   __syncthreads();
   int nbytes = 0;
   uint32_t x = kRansL;
-  if (act) {
-    const int nsym = count > r ? (count - r + 3) >> 2 : 0;  // symbols of this lane
-    uint8_t* lane_end = sbase + kScratchLanes + (r + 1) * kLaneBytes;
-    // ranks: aligned chunks ci, ci+1 in a, b; a block of 16 symbols takes
-    // the 16 bytes at offset `head` of a:b, while chunk ci+2 loads
-    const uint8_t* rp = sbase + kScratchRanks + 1024 * r + (1024 - nsym);
-    const int head = (int)((uintptr_t)rp & 15);
-    const q2* cp = (const q2*)(rp - head);
-    const int last = nsym > 0 ? (head + nsym - 1) >> 4 : 0;
-    const q2 ca = cp[0];
-    const q2 cb = cp[min(1, last)];
-    uint64_t a0 = ca.x, a1 = ca.y, b0 = cb.x, b1 = cb.y;
-    int ci = 0;
-    // byte queue: `pend` bytes, the most recent in the low byte of qlo
-    uint64_t qlo = 0, qhi = 0;
-    int pend = 0, blocks = 0;
-    uint64_t* my_sink = (uint64_t*)(sink + lane * 16);
-    __builtin_amdgcn_s_waitcnt(0);
-    for (int j0 = 0; j0 < nsym; j0 += 16) {
-      const uint64_t r_lo = window8_le(a0, a1, b0, head);
-      const uint64_t r_hi = window8_le(a1, b0, b1, head);
-      const q2 cn = cp[min(ci + 2, last)];
+  const int nsym = act && count > r ? (count - r + 3) >> 2 : 0;  // symbols of this lane
+  uint8_t* lane_end = sbase + kScratchLanes + (r + 1) * kLaneBytes;
+  // ranks: aligned chunks ci, ci+1 in a, b; a block of 16 symbols takes
+  // the 16 bytes at offset `head` of a:b, while chunk ci+2 loads
+  const uint8_t* rp = sbase + kScratchRanks + 1024 * r + (1024 - nsym);
+  const int head = (int)((uintptr_t)rp & 15);
+  const q2* cp = (const q2*)(rp - head);
+  const int last = nsym > 0 ? (head + nsym - 1) >> 4 : 0;
+  const q2 ca = cp[0];
+  const q2 cb = cp[min(1, last)];
+  uint64_t a0 = ca.x, a1 = ca.y, b0 = cb.x, b1 = cb.y;
+  int ci = 0;
+  // byte queue: `pend` (< 8) bytes, the most recent in the low byte of qq
+  uint64_t qq = 0;
+  int pend = 0, blocks = 0;
+  // byte blocks go through a buffer resource over the group's 16 scratch
+  // areas; rounds with nothing to store use an offset past it, which the
+  // hardware drops (a shared sink line would take every wave's stores)
+  const uint32_t grp_bytes =
+      (uint32_t)(min(shard_total - grp * kRansShards, kRansShards) * kScratchBytes);
+  const __amdgpu_buffer_rsrc_t lanes_r = __builtin_amdgcn_make_buffer_rsrc(
+      scratch + (int64_t)grp * kRansShards * kScratchBytes, 0, (int)grp_bytes, 0x00020000);
+  const uint32_t lane_end_off =
+      (uint32_t)((lane >> 2) * kScratchBytes + kScratchLanes + (r + 1) * kLaneBytes);
+  // rounds run unpredicated while every active lane of the wave has 16 more
+  // symbols; inactive lanes compute nothing of use and store nothing
+  // (a group without active lanes runs no rounds at all)
+  const int wave_j = __builtin_amdgcn_readfirstlane((int)wave_max((uint32_t)nsym));
+  const int full_j = min(wave_j, __builtin_amdgcn_readfirstlane(
+                                     (int)wave_min((uint32_t)(act ? nsym : 0x7fffffff)))) & ~15;
+  __builtin_amdgcn_s_waitcnt(0);
+
+  // one block of 16 symbols: four rounds of four steps.  A step emits its
+  // renormalisation bytes into the round's word `wd` (most recent lowest);
+  // the round then merges them into the queue and stores the oldest eight
+  // queued bytes when there are eight.
+  auto block = [&](int j0, bool predicated) __attribute__((always_inline)) {
+    const uint64_t r_lo = window8_le(a0, a1, b0, head);
+    const uint64_t r_hi = window8_le(a1, b0, b1, head);
+    const q2 cn = cp[min(ci + 2, last)];
 #pragma unroll
-      for (int rd = 0; rd < 4; ++rd) {
-        const uint32_t rk4 = (uint32_t)((rd < 2 ? r_lo : r_hi) >> (32 * (rd & 1)));
-        uint2 e[4];
+    for (int rd = 0; rd < 4; ++rd) {
+      const uint32_t rk4 = (uint32_t)((rd < 2 ? r_lo : r_hi) >> (32 * (rd & 1)));
+      uint4 e[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) e[t] = ent[(rk4 >> (8 * t)) & 0xffu];
+      for (int t = 0; t < 4; ++t) e[t] = ent[(rk4 >> (8 * t)) & 0xffu];
+      uint64_t wd = 0;
+      uint32_t wbits = 0;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const bool live = j0 + 4 * rd + t < nsym;
-          const uint32_t freq = e[t].x & 0x1fffu;
-          const uint32_t cum = (e[t].x >> 13) & 0xfffu;
-          const uint32_t c = e[t].x >> 25;
-          const uint32_t lim = freq << 19;
-          const int nb = live ? (int)(x >= lim) + (int)((x >> 8) >= lim) : 0;
-          // append the nb low bytes of x, first the lowest
-          const uint32_t sh = 8u * (uint32_t)nb;
-          const uint32_t add = nb == 2 ? (((x & 0xffu) << 8) | ((x >> 8) & 0xffu)) : (x & 0xffu);
-          qhi = sh ? (qhi << sh) | (qlo >> (64u - sh)) : qhi;
-          qlo = sh ? (qlo << sh) | add : qlo;
-          pend += nb;
-          const uint32_t xs = x >> sh;
-          const uint32_t q = c ? (__umulhi(xs, e[t].y) >> (c - 1)) : xs;
-          const uint32_t xn = xs + __umul24(q, 4096u - freq) + cum;
-          x = live ? xn : x;
-        }
-        // the oldest eight pending bytes, if there are eight, in memory order
-        const bool full = pend >= 8;
-        const uint32_t ds = 8u * (uint32_t)(pend - 8);  // < 64 when full
-        const uint64_t word = full ? (ds ? (qlo >> ds) | (qhi << (64u - ds)) : qlo) : 0ull;
-        const bool fits = 8 * (blocks + 1) <= kLaneBytes;
-        int st_ok = full && fits;
-        asm volatile("" : "+v"(st_ok));
-        *(st_ok ? (uint64_t*)(lane_end - 8 * (blocks + 1)) : my_sink) = word;
-        blocks += full ? 1 : 0;
-        pend -= full ? 8 : 0;
+      for (int t = 0; t < 4; ++t) {
+        const bool live = !predicated || j0 + 4 * rd + t < nsym;
+        const uint32_t lim = e[t].x;
+        uint32_t sh = (x >> 8) >= lim ? 16u : x >= lim ? 8u : 0u;
+        if (!live) sh = 0;
+        // the bytes in emission order, low then high: byte-swapped low half
+        const uint32_t pv = __builtin_amdgcn_perm(0u, x, 0x0c0c0001u);
+        wd = (wd << sh) | (pv >> (16u - sh));
+        wbits += sh;
+        const uint32_t xs = x >> sh;
+        const double rc = __longlong_as_double((int64_t)(((uint64_t)e[t].w << 32) | e[t].z));
+        const uint32_t q = (uint32_t)((double)xs * rc);
+        const uint32_t xn = xs + (e[t].y & 0xfffu) + __umul24(q, e[t].y >> 12);
+        x = live ? xn : x;
       }
-      a0 = b0;
-      a1 = b1;
-      b0 = cn.x;
-      b1 = cn.y;
-      ++ci;
+      const int total = pend + (int)(wbits >> 3);
+      const bool full = total >= 8;
+      const uint32_t d = (uint32_t)(8 * (total - 8)) & 63u;
+      // the oldest eight bytes: the queue's pend bytes above the round's
+      const uint64_t word = (qq << ((64u - 8u * (uint32_t)pend) & 63u)) | (wd >> d);
+      const uint64_t keep = wd & ((1ull << d) - 1ull);
+      const uint64_t merged = (qq << (wbits & 63u)) | wd;
+      qq = full ? keep : merged;
+      pend = full ? total - 8 : total;
+      const bool fits = 8 * (blocks + 1) <= kLaneBytes;
+      const uint32_t off = full && fits && act ? lane_end_off - 8u * (uint32_t)(blocks + 1)
+                                               : 0x80000000u;
+      using u32x2 = uint32_t __attribute__((ext_vector_type(2)));
+      __builtin_amdgcn_raw_buffer_store_b64(u32x2{(uint32_t)word, (uint32_t)(word >> 32)}, lanes_r,
+                                            off, 0, 0);
+      blocks += full ? 1 : 0;
     }
+    a0 = b0;
+    a1 = b1;
+    b0 = cn.x;
+    b1 = cn.y;
+    ++ci;
+  };
+  int j0 = 0;
+  for (; j0 < full_j; j0 += 16) block(j0, false);
+  for (; j0 < wave_j; j0 += 16) block(j0, true);
+  if (act) {
     // the last pend (< 8) bytes: byte i of the queue (i = 0 most recent) is
     // emitted byte 8 blocks + pend - 1 - i, stored at lane_end - 1 - that
     nbytes = 8 * blocks + pend;
     for (int i = 0; i < pend; ++i) {
       const int e_i = 8 * blocks + pend - 1 - i;
-      if (e_i < kLaneBytes) lane_end[-1 - e_i] = (uint8_t)(qlo >> (8 * i));
+      if (e_i < kLaneBytes) lane_end[-1 - e_i] = (uint8_t)(qq >> (8 * i));
     }
     if (nbytes > kLaneBytes) set_status(st, kStatusLaneOverflow);
     recs[s].states[r] = x;
   }
   const int base = lane & ~3;
-  const int b0 = shfl(nbytes, base);
-  const int b1 = shfl(nbytes, base + 1);
-  const int b2 = shfl(nbytes, base + 2);
-  const int b3 = shfl(nbytes, base + 3);
+  const int n0 = shfl(nbytes, base);
+  const int n1 = shfl(nbytes, base + 1);
+  const int n2 = shfl(nbytes, base + 2);
+  const int n3 = shfl(nbytes, base + 3);
   if (act && r == 0) {
     ShardRec& rec = recs[s];
-    const uint32_t body0 = (uint32_t)rec.sparse_len + 28u + (uint32_t)(b0 + b1 + b2 + b3);
+    const uint32_t body0 = (uint32_t)rec.sparse_len + 28u + (uint32_t)(n0 + n1 + n2 + n3);
     if (frames[td.frame].force_order0 || body0 < rec.best3_body) {
       rec.mode = kModeOrder0;
       rec.body = body0;
-      rec.lane[0] = (uint16_t)b0;
-      rec.lane[1] = (uint16_t)b1;
-      rec.lane[2] = (uint16_t)b2;
-      rec.lane[3] = (uint16_t)b3;
+      rec.lane[0] = (uint16_t)n0;
+      rec.lane[1] = (uint16_t)n1;
+      rec.lane[2] = (uint16_t)n2;
+      rec.lane[3] = (uint16_t)n3;
     }
   }
 }
@@ -951,7 +997,7 @@ __global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frame
                                              uint8_t* __restrict__ scratch,
                                              uint8_t* __restrict__ sink, int cap_below) {
   (void)folded;
-  __shared__ uint2 ent_all[kRansShards][kCap];
+  __shared__ uint4 ent_all[kRansShards][kCap];
   const int groups = (shard_total + kRansShards - 1) / kRansShards;
   for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
     rans_group<FT>(frames, states, tiles, shards, shard_total, recs, scratch, sink,
