@@ -114,8 +114,8 @@ Geometry fill_geometry(FrameDesc& f, int64_t width, int64_t height, int64_t tw, 
 }
 
 struct WorkspaceLayout {
-  int64_t frames, states, tiles, shards, recs, tile_len, tile_off, folded, scratch, recon, dshards,
-      sink, queues, groups;
+  int64_t frames, states, tiles, shards, recs, tile_len, tile_off, folded, scratch, ranks, recon,
+      dshards, sink, queues, groups;
   int64_t total;
 };
 
@@ -307,6 +307,7 @@ int build_encode_plan(const fvh_encode_frame* in, int32_t count, EncodePlan& pla
   w.tile_off = take(plan.tiles * 8);
   w.folded = take((plan.samples + 8) * (plan.wide_folded ? 4 : 2));
   w.scratch = take(plan.shards * (int64_t)kScratchBytes);
+  w.ranks = take((plan.shards + kRansShards - 1) / kRansShards * (int64_t)kRankGroupBytes);
   w.recon = plan.any_wide ? take(plan.samples * 2) : -1;
   w.dshards = -1;
   w.sink = take(kSinkBytes);
@@ -361,6 +362,7 @@ int fvh_encode_launch(const fvh_encode_frame* frames, int32_t frame_count, void*
   b.tile_off = (int64_t*)(ws + plan.ws.tile_off);
   b.folded = ws + plan.ws.folded;
   b.scratch = ws + plan.ws.scratch;
+  b.ranks = ws + plan.ws.ranks;
   b.recon = plan.ws.recon >= 0 ? (uint16_t*)(ws + plan.ws.recon) : nullptr;
   b.sink = ws + plan.ws.sink;
   b.queues = (int32_t*)(ws + plan.ws.queues);

@@ -34,12 +34,18 @@ constexpr int kMaxRecordBytes = 8739;
 // Per-shard order-0 scratch layout (bytes).
 constexpr int kScratchTable = 0;        // 256 x {freq|cum<<13|c<<25, reciprocal} by rank
 constexpr int kScratchSparse = 2048;    // sparse table bytes (<= 1024)
-constexpr int kScratchRanks = 3072;     // ranks, 1024 bytes per rANS lane r: the rank of symbol
-                                        // 4k + r at byte 1024 r + 1023 - k (the order the
-                                        // encoder visits them)
-constexpr int kScratchLanes = 7168;     // 4 lanes x kLaneBytes, filled from the end
+constexpr int kScratchLanes = 3072;     // 4 lanes x kLaneBytes, filled from the end
 constexpr int kLaneBytes = 1536;        // proven bound for 1024 symbols
-constexpr int kScratchBytes = kScratchLanes + 4 * kLaneBytes;  // 13312
+constexpr int kScratchBytes = kScratchLanes + 4 * kLaneBytes;  // 9216
+
+// Order-0 rank bytes, by group of kRansShards consecutive shards.  rANS lane
+// r of shard s is lane L = 4 (s % kRansShards) + r of its group; its 1024
+// rank bytes (byte p: the rank of symbol 4 (1023 - p) + r, the order the
+// encoder visits them) are 64 chunks of 16, chunk q at byte (64 q + L) * 16
+// of the group's area, so one chunk index of all 64 lanes is one contiguous
+// kilobyte.
+constexpr int kRansShards = 16;
+constexpr int kRankGroupBytes = kRansShards * kShardSymbols;  // 65536
 
 // Frame-level status codes (first error wins).
 constexpr int kStatusSampleRange = 1;     // encoder: sample > max (encode.cu:160-162)

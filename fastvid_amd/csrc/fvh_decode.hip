@@ -766,10 +766,10 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
   uint32_t* const ent = lds.row[g];
   uint8_t* const prefix = lds.prefix[g];
   FT* const stage = (FT*)lds.stage[g];
-  for (;;) {
-    int grp = 0;
-    if (lane == 0) grp = atomicAdd(queue, 1);
-    grp = __builtin_amdgcn_readfirstlane(grp);
+  // a wave's first group is its block index, later ones come from the
+  // queue: blocks past the list's end (all of them, for an empty tier) leave
+  // without touching the queue counter every wave shares
+  for (int grp = blockIdx.x;;) {
     if (grp * kO0Shards >= count) break;
     const int at = grp * kO0Shards + g;
     const int item = at < count ? tier_list[at] : -1;
@@ -940,6 +940,9 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
       flush(s0);
     }
     if (nsym > 0 && (consumed != 8 * length || x != kRansL)) set_status(&states[d.frame], 5);
+    int next = 0;
+    if (lane == 0) next = atomicAdd(queue, 1);
+    grp = (int)gridDim.x + __builtin_amdgcn_readfirstlane(next);
   }
 }
 

@@ -462,7 +462,8 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
                                                  const ShardDesc* __restrict__ shards,
                                                  int shard_total, const FT* __restrict__ folded,
                                                  ShardRec* __restrict__ recs,
-                                                 uint8_t* __restrict__ scratch) {
+                                                 uint8_t* __restrict__ scratch,
+                                                 uint8_t* __restrict__ rank_area) {
   __shared__ uint32_t hist_all[4][512];
   __shared__ uint64_t zw_all[4][64];
   const int lane = lane_id();
@@ -648,23 +649,26 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
       rec.sparse_len = (uint16_t)(hdr + shfl(len_incl, 63));
       rec.distinct = (uint16_t)distinct;
       rec.supported = 1;
-      // rank stream, one region per rANS lane r (symbols 4k + r) holding the
-      // rank of symbol 4k + r at byte 1023 - k: the rANS lanes visit symbols
-      // from the last one back, so each reads its region forwards.  Symbols
-      // b = r and b = r + 4 of a chunk are k and k + 1 of lane r, one 16-bit
-      // store at byte 1022 - k.
+      // rank bytes of rANS lane r (symbols 4k + r): the rank of symbol 4k + r
+      // at byte p = 1023 - k, in the group-interleaved chunks of the rank
+      // area (the rANS lanes visit symbols from the last one back, so each
+      // reads its bytes forwards).  Symbols b = r and b = r + 4 of a chunk
+      // are k and k + 1 of lane r: one 16-bit store at byte 1022 - k, which
+      // is even, so both bytes fall in one 16-byte chunk.
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int c = 0; c < 8; ++c) ch[c].launder();
-      uint8_t* ranks = sbase + kScratchRanks;
+      uint8_t* ranks = rank_area + (int64_t)(s / kRansShards) * kRankGroupBytes +
+                       16 * 4 * (s % kRansShards);
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
-        const int k = 128 * c + 2 * lane;  // lane r index of symbol b = r
+        const int p = 1022 - (128 * c + 2 * lane);  // byte of symbol b = r + 4 (lane r)
+        uint8_t* const at = ranks + 1024 * (p >> 4) + (p & 15);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const uint32_t rk0 = hist[ch[c].get(r)] & 0xffu;
           const uint32_t rk1 = hist[ch[c].get(r + 4)] & 0xffu;
-          *(uint16_t*)(ranks + 1024 * r + 1022 - k) = (uint16_t)(rk1 | (rk0 << 8));
+          *(uint16_t*)(at + 16 * r) = (uint16_t)(rk1 | (rk0 << 8));
         }
       }
     }
@@ -786,8 +790,9 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
 // (4096 - freq) << 12, 1/freq rounded up to a double}: the renormalisation
 // test is two compares, and floor(x / freq) is trunc(double(x) * 1/freq),
 // exact for every state the encoder divides (x < freq * 2^19 <= 2^31).  A
-// lane reads its rank region forwards, 16 symbols per block from a two-chunk
-// window whose next chunk loads while the block runs.  A step's
+// lane reads its rank bytes forwards, 16 symbols per block from a two-chunk
+// window whose next chunk loads while the block runs; the group-interleaved
+// rank layout makes each such load one contiguous kilobyte for the wave.  A step's
 // renormalisation bytes go into the round's 64-bit word, a round (four
 // steps) merges them into a queue of fewer than eight bytes, and when eight
 // are queued it stores the oldest eight to the end of the lane's scratch
@@ -803,7 +808,6 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
 // whose largest alphabet falls in its range (the others skip them), so the
 // LDS held per wave follows the alphabets actually present: 4 KB for small
 // alphabets, which lets a compute unit hold every wave its registers allow.
-constexpr int kRansShards = 16;
 
 // Eight bytes starting at byte `pos` (0..15) of the 32-byte window a0:a1:b0:b1,
 // the first byte least significant.
@@ -822,8 +826,8 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
                                            const ShardDesc* __restrict__ shards, int shard_total,
                                            ShardRec* __restrict__ recs,
                                            uint8_t* __restrict__ scratch,
-                                           uint8_t* __restrict__ sink, uint4* ent, int grp,
-                                           int cap, int cap_below) {
+                                           const uint8_t* __restrict__ rank_area, uint4* ent,
+                                           int grp, int cap, int cap_below) {
   using q2 = uint64_t __attribute__((ext_vector_type(2)));
   const int lane = lane_id();
   const int r = lane & 3;
@@ -869,14 +873,17 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   uint32_t x = kRansL;
   const int nsym = act && count > r ? (count - r + 3) >> 2 : 0;  // symbols of this lane
   uint8_t* lane_end = sbase + kScratchLanes + (r + 1) * kLaneBytes;
-  // ranks: aligned chunks ci, ci+1 in a, b; a block of 16 symbols takes
-  // the 16 bytes at offset `head` of a:b, while chunk ci+2 loads
-  const uint8_t* rp = sbase + kScratchRanks + 1024 * r + (1024 - nsym);
-  const int head = (int)((uintptr_t)rp & 15);
-  const q2* cp = (const q2*)(rp - head);
+  // ranks: the lane's bytes start at byte 1024 - nsym of its 1024; aligned
+  // chunks ci, ci+1 in a, b (chunk i at cp[64 i]: the group's lanes'
+  // chunks interleave); a block of 16 symbols takes the 16 bytes at offset
+  // `head` of a:b, while chunk ci+2 loads.  A lane without symbols reads
+  // its first chunk and never uses it.
+  const int p0 = nsym > 0 ? 1024 - nsym : 0;
+  const int head = p0 & 15;
+  const q2* cp = (const q2*)(rank_area + (int64_t)grp * kRankGroupBytes) + 64 * (p0 >> 4) + lane;
   const int last = nsym > 0 ? (head + nsym - 1) >> 4 : 0;
   const q2 ca = cp[0];
-  const q2 cb = cp[min(1, last)];
+  const q2 cb = cp[64 * min(1, last)];
   uint64_t a0 = ca.x, a1 = ca.y, b0 = cb.x, b1 = cb.y;
   int ci = 0;
   // byte queue: `pend` (< 8) bytes, the most recent in the low byte of qq
@@ -906,7 +913,7 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   auto block = [&](int j0, bool predicated) __attribute__((always_inline)) {
     const uint64_t r_lo = window8_le(a0, a1, b0, head);
     const uint64_t r_hi = window8_le(a1, b0, b1, head);
-    const q2 cn = cp[min(ci + 2, last)];
+    const q2 cn = cp[64 * min(ci + 2, last)];
 #pragma unroll
     for (int rd = 0; rd < 4; ++rd) {
       const uint32_t rk4 = (uint32_t)((rd < 2 ? r_lo : r_hi) >> (32 * (rd & 1)));
@@ -995,12 +1002,13 @@ __global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frame
                                              int shard_total, const FT* __restrict__ folded,
                                              ShardRec* __restrict__ recs,
                                              uint8_t* __restrict__ scratch,
-                                             uint8_t* __restrict__ sink, int cap_below) {
+                                             const uint8_t* __restrict__ rank_area,
+                                             int cap_below) {
   (void)folded;
   __shared__ uint4 ent_all[kRansShards][kCap];
   const int groups = (shard_total + kRansShards - 1) / kRansShards;
   for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
-    rans_group<FT>(frames, states, tiles, shards, shard_total, recs, scratch, sink,
+    rans_group<FT>(frames, states, tiles, shards, shard_total, recs, scratch, rank_area,
                    ent_all[lane_id() >> 2], grp, kCap, cap_below);
   }
 }
@@ -1356,7 +1364,7 @@ static void encode_pass(const EncodeBuffers& b, int pass, hipStream_t stream) {
     FVH_TIMED("k_analyze", stream);
     hipLaunchKernelGGL((k_analyze<FT>), dim3((b.shard_total + 3) / 4), dim3(256), 0, stream,
                        b.frames, b.states, b.tiles, b.shards, b.shard_total, folded, b.recs,
-                       b.scratch);
+                       b.scratch, b.ranks);
   }
   {
     FVH_TIMED("k_rans", stream);
@@ -1364,11 +1372,11 @@ static void encode_pass(const EncodeBuffers& b, int pass, hipStream_t stream) {
     const int groups = (b.shard_total + kRansShards - 1) / kRansShards;
     const dim3 grid(min(groups, 32 * device_cus()));
     hipLaunchKernelGGL((k_rans<FT, 16>), grid, dim3(64), 0, stream, b.frames, b.states, b.tiles,
-                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.sink, -1);
+                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.ranks, -1);
     hipLaunchKernelGGL((k_rans<FT, 64>), grid, dim3(64), 0, stream, b.frames, b.states, b.tiles,
-                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.sink, 16);
+                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.ranks, 16);
     hipLaunchKernelGGL((k_rans<FT, 256>), grid, dim3(64), 0, stream, b.frames, b.states, b.tiles,
-                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.sink, 64);
+                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.ranks, 64);
   }
 }
 

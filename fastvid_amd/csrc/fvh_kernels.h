@@ -21,6 +21,7 @@ struct EncodeBuffers {
   int64_t* tile_off;
   void* folded;          // uint16 or uint32 residuals, tile order
   uint8_t* scratch;      // kScratchBytes per shard
+  uint8_t* ranks;        // kRankGroupBytes per group of kRansShards shards
   uint16_t* recon;       // tile-order reconstruction for tiles wider than kRingMax (or null)
   uint8_t* sink;         // kSinkBytes target of masked-off stores
   int32_t* queues;       // kQueueCount work counters, zeroed by k_state_init
