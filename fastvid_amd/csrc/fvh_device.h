@@ -22,41 +22,81 @@ __device__ __forceinline__ T shfl_up(T v, int delta) { return __shfl_up(v, delta
 template <typename T>
 __device__ __forceinline__ T shfl_xor(T v, int mask) { return __shfl_xor(v, mask, 64); }
 
+// Cross-lane moves in the VALU (DPP, gfx9 controls) instead of LDS permutes.
+// row_shr lanes whose source lies before their 16-lane row read 0; rows
+// outside kRowMask keep `old`.
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t old, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, kCtrl, kRowMask, 0xf, true);
+}
+constexpr int kDppXor1 = 0xb1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4e;        // quad_perm [2,3,0,1]
+constexpr int kDppRowShr = 0x110;     // + n: row_shr:n
+constexpr int kDppMirror = 0x140;     // lane i of a row <- lane 15 - i
+constexpr int kDppHalfMirror = 0x141; // lane i of a half row <- lane 7 - i
+constexpr int kDppBcast15 = 0x142;    // lane 15 of each row -> the next row
+constexpr int kDppBcast31 = 0x143;    // lane 31 -> rows 2 and 3
+
+// Reduction of a commutative, associative op over the 64 lanes, returned
+// uniform: quads, half rows and rows combine in place, then the row
+// broadcasts carry rows 0..2 into lane 63.  `identity` fills rows the
+// broadcasts do not write.  The whole wave must be active.
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_reduce(uint32_t v, uint32_t identity, Op op) {
+  v = op(v, dpp_mov<kDppXor1>(identity, v));
+  v = op(v, dpp_mov<kDppXor2>(identity, v));
+  v = op(v, dpp_mov<kDppHalfMirror>(identity, v));
+  v = op(v, dpp_mov<kDppMirror>(identity, v));
+  v = op(v, dpp_mov<kDppBcast15, 0xa>(identity, v));
+  v = op(v, dpp_mov<kDppBcast31, 0xc>(identity, v));
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v |= shfl_xor(v, m);
-  return v;
+  return wave_reduce(v, 0u, [](uint32_t a, uint32_t b) { return a | b; });
 }
 
 __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v = min(v, (uint32_t)shfl_xor(v, m));
-  return v;
+  return wave_reduce(v, ~0u, [](uint32_t a, uint32_t b) { return min(a, b); });
 }
 
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v = max(v, (uint32_t)shfl_xor(v, m));
-  return v;
+  return wave_reduce(v, 0u, [](uint32_t a, uint32_t b) { return max(a, b); });
 }
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return (T)wave_reduce((uint32_t)v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+  } else {
 #pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v += shfl_xor(v, m);
-  return v;
+    for (int m = 32; m > 0; m >>= 1) v += shfl_xor(v, m);
+    return v;
+  }
 }
 
-// Inclusive prefix sum over the 64 lanes.
+// Inclusive prefix sum over the 64 lanes (whole wave active).  32-bit
+// values: Hillis-Steele within rows by row shifts, then the row broadcasts.
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
-  const int lane = lane_id();
+  if constexpr (sizeof(T) == 4) {
+    uint32_t x = (uint32_t)v;
+    x += dpp_mov<kDppRowShr + 1>(0u, x);
+    x += dpp_mov<kDppRowShr + 2>(0u, x);
+    x += dpp_mov<kDppRowShr + 4>(0u, x);
+    x += dpp_mov<kDppRowShr + 8>(0u, x);
+    x += dpp_mov<kDppBcast15, 0xa>(0u, x);
+    x += dpp_mov<kDppBcast31, 0xc>(0u, x);
+    return (T)x;
+  } else {
+    const int lane = lane_id();
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const T o = shfl_up(v, d);
-    if (lane >= d) v += o;
+    for (int d = 1; d < 64; d <<= 1) {
+      const T o = shfl_up(v, d);
+      if (lane >= d) v += o;
+    }
+    return v;
   }
-  return v;
 }
 
 __device__ __forceinline__ int bit_length(uint32_t v) {

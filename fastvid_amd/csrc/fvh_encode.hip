@@ -1129,7 +1129,9 @@ __global__ __launch_bounds__(256) void k_frame_finish(const FrameDesc* __restric
 
 // ------------------------------------------------------------- emission
 
-constexpr int kBufWords = (kMaxRecordBytes + 3) / 4 + 2;
+// a record, shifted by up to 15 bytes to its 16-byte phase, plus the word
+// or_bits may touch past it, in whole 16-byte chunks
+constexpr int kBufWords = ((kMaxRecordBytes + 15 + 3) / 4 + 1 + 3) & ~3;
 
 __device__ __forceinline__ void or_bits(uint32_t* buf, uint32_t pos, uint32_t pattern) {
   const uint64_t pp = (uint64_t)pattern << (pos & 31);
@@ -1140,7 +1142,10 @@ __device__ __forceinline__ void or_bits(uint32_t* buf, uint32_t pos, uint32_t pa
 
 // One wave64 per shard builds the record [mode][body u16][body] in LDS and
 // copies it to its offset; the reference instead sets bits with global
-// atomics into a zero-filled stream (encode.cu:458-474, 906).
+// atomics into a zero-filled stream (encode.cu:458-474, 906).  The record
+// sits in LDS at its destination's offset within a 16-byte chunk, so the
+// copy is one 16-byte store per chunk it fills, and byte stores only in the
+// first and last chunks, which it shares with the neighbouring records.
 template <typename FT>
 __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ frames,
                                               const FrameState* __restrict__ states,
@@ -1150,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
                                               const ShardRec* __restrict__ recs,
                                               const uint8_t* __restrict__ scratch,
                                               const int64_t* __restrict__ tile_off, char* gbase) {
-  __shared__ uint32_t buf_all[4][kBufWords];
+  __shared__ __align__(16) uint32_t buf_all[4][kBufWords];
   __shared__ uint64_t zm_all[4][65];
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
@@ -1166,10 +1171,12 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
   const FT* v = folded + td.folded_base + local0;
   uint8_t* out = as_global(gbase, f.stream + tile_off[sd.tile] + rec.offset);
   const int total = 3 + (int)rec.body;
+  const int sh = (int)((uintptr_t)out & 15);
+  const uint32_t hdr = 3u + (uint32_t)sh;  // body's first byte in buf
   uint32_t* buf = buf_all[wave];
-  uint8_t* bytes = (uint8_t*)buf;
+  uint8_t* bytes = (uint8_t*)buf + sh;
   uint8_t* body = bytes + 3;
-  const int words = (total + 3) >> 2;
+  const int words = (sh + total + 3) >> 2;
   for (int i = lane; i < words + 1; i += 64) buf[i] = 0;
   __builtin_amdgcn_wave_barrier();
   const int mode = rec.mode;
@@ -1258,7 +1265,7 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
       const uint32_t carry = cls == 0 ? c0 : cls == 1 ? c1 : cls == 2 ? c2 : c3;
       if (valid) {
         const uint32_t start = carry + incl - len;
-        const uint32_t pos = (3u + lane_off[cls]) * 8u + start + qv;
+        const uint32_t pos = (hdr + lane_off[cls]) * 8u + start + qv;
         or_bits(buf, pos, 1u | ((x & rmask) << 1));
       }
       c0 += shfl(incl, 60);
@@ -1294,7 +1301,7 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
       const int boff = shfl(off, b);
       if (i < count && w > 0) {
         const uint32_t li = (uint32_t)(((j & 1) << 6) + lane);
-        const uint32_t pos = (3u + (uint32_t)boff + 1u) * 8u + li * (uint32_t)w;
+        const uint32_t pos = (hdr + (uint32_t)boff + 1u) * 8u + li * (uint32_t)w;
         or_bits(buf, pos, (uint32_t)v[i]);
       }
     }
@@ -1303,7 +1310,36 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
     // states, lane bytes in decode order.
     const uint8_t* sbase = scratch + (int64_t)s * kScratchBytes;
     const int sp = rec.sparse_len;
-    for (int i = lane; i < sp; i += 64) body[i] = sbase[kScratchSparse + i];
+    // body bytes [0, sp) are the sparse table and [d1, d5) the four lanes'
+    // bytes, lane r's ending at the end of its scratch area: one gather with
+    // sixteen independent loads in flight per thread (the 28 bytes between
+    // come from the record)
+    const int d1 = sp + 28;
+    const int d2 = d1 + rec.lane[0];
+    const int d3 = d2 + rec.lane[1];
+    const int d4 = d3 + rec.lane[2];
+    const int d5 = d4 + rec.lane[3];
+    const int src_off[5] = {kScratchSparse, kScratchLanes + 1 * kLaneBytes - d2,
+                            kScratchLanes + 2 * kLaneBytes - d3,
+                            kScratchLanes + 3 * kLaneBytes - d4,
+                            kScratchLanes + 4 * kLaneBytes - d5};
+    for (int b0 = 0; b0 < d5; b0 += 1024) {
+      uint32_t val[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int i = b0 + 64 * u + lane;
+        const int piece = (i >= d1) + (i >= d2) + (i >= d3) + (i >= d4);
+        const int so = piece == 0 ? src_off[0] : piece == 1 ? src_off[1] : piece == 2 ? src_off[2]
+                       : piece == 3 ? src_off[3] : src_off[4];
+        const bool in = i < sp || (i >= d1 && i < d5);
+        val[u] = in ? sbase[so + i] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int i = b0 + 64 * u + lane;
+        if (i < sp || (i >= d1 && i < d5)) body[i] = (uint8_t)val[u];
+      }
+    }
     if (lane < 3) {
       const uint32_t len = rec.lane[lane];
       for (int q = 0; q < 4; ++q) body[sp + lane * 4 + q] = (uint8_t)(len >> (8 * q));
@@ -1311,13 +1347,6 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
     if (lane < 4) {
       const uint32_t stt = rec.states[lane];
       for (int q = 0; q < 4; ++q) body[sp + 12 + lane * 4 + q] = (uint8_t)(stt >> (8 * q));
-    }
-    int off = sp + 28;
-    for (int r = 0; r < 4; ++r) {
-      const int len = rec.lane[r];
-      const uint8_t* srcl = sbase + kScratchLanes + (r + 1) * kLaneBytes - len;
-      for (int i = lane; i < len; i += 64) body[off + i] = srcl[i];
-      off += len;
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -1327,7 +1356,17 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
     bytes[2] = (uint8_t)(rec.body >> 8);
   }
   __builtin_amdgcn_wave_barrier();
-  for (int i = lane; i < total; i += 64) out[i] = bytes[i];
+  uint8_t* const out16 = out - sh;
+  const int end = sh + total;
+  for (int c = lane; 16 * c < end; c += 64) {
+    if (16 * c >= sh && 16 * c + 16 <= end) {
+      ((uint4*)out16)[c] = ((const uint4*)buf)[c];
+    } else {
+      for (int j = max(sh - 16 * c, 0); j < 16 && 16 * c + j < end; ++j) {
+        out16[16 * c + j] = ((const uint8_t*)buf)[16 * c + j];
+      }
+    }
+  }
 }
 
 // ------------------------------------------------------------ orchestration
