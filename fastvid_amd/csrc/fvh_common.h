@@ -32,7 +32,7 @@ constexpr uint32_t kRansL = 1u << 23;
 constexpr int kMaxRecordBytes = 8739;
 
 // Per-shard order-0 scratch layout (bytes).
-constexpr int kScratchTable = 0;        // 256 x {freq|cum<<13|c<<25, reciprocal} by rank
+constexpr int kScratchTable = 0;        // 256 x {freq | cum << 13, 0} by rank
 constexpr int kScratchSparse = 2048;    // sparse table bytes (<= 1024)
 constexpr int kScratchLanes = 3072;     // 4 lanes x kLaneBytes, filled from the end
 constexpr int kLaneBytes = 1536;        // proven bound for 1024 symbols

@@ -23,19 +23,41 @@ template <typename T>
 __device__ __forceinline__ T shfl_xor(T v, int mask) { return __shfl_xor(v, mask, 64); }
 
 // Cross-lane moves in the VALU (DPP, gfx9 controls) instead of LDS permutes.
-// row_shr lanes whose source lies before their 16-lane row read 0; rows
-// outside kRowMask keep `old`.
-template <int kCtrl, int kRowMask = 0xf>
+// A lane whose source lies outside the move's range (before its 16-lane row
+// for row_shr, past the wave for the wave shifts) reads 0, or keeps `old`
+// when kZero is false; rows outside kRowMask keep `old`.
+template <int kCtrl, int kRowMask = 0xf, bool kZero = true>
 __device__ __forceinline__ uint32_t dpp_mov(uint32_t old, uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, kCtrl, kRowMask, 0xf, true);
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, kCtrl, kRowMask, 0xf, kZero);
 }
 constexpr int kDppXor1 = 0xb1;        // quad_perm [1,0,3,2]
 constexpr int kDppXor2 = 0x4e;        // quad_perm [2,3,0,1]
 constexpr int kDppRowShr = 0x110;     // + n: row_shr:n
+constexpr int kDppWaveShl1 = 0x130;   // lane i <- lane i + 1
+constexpr int kDppWaveShr1 = 0x138;   // lane i <- lane i - 1
 constexpr int kDppMirror = 0x140;     // lane i of a row <- lane 15 - i
 constexpr int kDppHalfMirror = 0x141; // lane i of a half row <- lane 7 - i
 constexpr int kDppBcast15 = 0x142;    // lane 15 of each row -> the next row
 constexpr int kDppBcast31 = 0x143;    // lane 31 -> rows 2 and 3
+
+// The previous lane's value (0 on lane 0) and the next lane's (0 on lane 63).
+__device__ __forceinline__ uint32_t from_prev_lane(uint32_t v) { return dpp_mov<kDppWaveShr1>(0u, v); }
+__device__ __forceinline__ uint32_t from_next_lane(uint32_t v) { return dpp_mov<kDppWaveShl1>(0u, v); }
+__device__ __forceinline__ uint64_t from_prev_lane(uint64_t v) {
+  return ((uint64_t)from_prev_lane((uint32_t)(v >> 32)) << 32) | from_prev_lane((uint32_t)v);
+}
+__device__ __forceinline__ uint64_t from_next_lane(uint64_t v) {
+  return ((uint64_t)from_next_lane((uint32_t)(v >> 32)) << 32) | from_next_lane((uint32_t)v);
+}
+
+// OR over each 16-lane row, in every lane of the row.
+__device__ __forceinline__ uint32_t row_or(uint32_t v) {
+  v |= dpp_mov<kDppXor1>(0u, v);
+  v |= dpp_mov<kDppXor2>(0u, v);
+  v |= dpp_mov<kDppHalfMirror>(0u, v);
+  v |= dpp_mov<kDppMirror>(0u, v);
+  return v;
+}
 
 // Reduction of a commutative, associative op over the 64 lanes, returned
 // uniform: quads, half rows and rows combine in place, then the row

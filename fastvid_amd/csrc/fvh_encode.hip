@@ -362,24 +362,6 @@ __device__ __forceinline__ ClampFn norm_fn(uint32_t prefix, int count, int remai
   return ClampFn{1, e, u};
 }
 
-// rANS encoder entry of one symbol, indexed by its rank among the shard's
-// present values: x = freq (13 bits) | cum << 13 (12 bits) | c << 25 (4 bits),
-// y = m, with c = ceil(log2 freq) and m = ceil(2^(31+c) / freq):
-// floor(x / freq) = mul_hi(x, m) >> (c-1) for every state x < 2^31 (freq >= 2;
-// freq == 1 divides trivially).
-// m is computed as a double quotient: 2^(31+c) / freq lies in [2^31, 2^32)
-// and, when not an integer, is at least 1/freq >= 2^-12 from one, while the
-// quotient's rounding error is at most 2^-22, so its ceiling is exact.
-__device__ __forceinline__ uint2 rans_entry(uint32_t freq, uint32_t cum) {
-  uint32_t c = 0;
-  uint32_t m = 0;
-  if (freq >= 2) {
-    c = 32 - __builtin_clz(freq - 1);
-    m = (uint32_t)ceil((double)(1ull << (31 + c)) / (double)freq);
-  }
-  return make_uint2(freq | (cum << 13) | (c << 25), m);
-}
-
 __device__ __forceinline__ int write_varint_bytes(uint8_t* out, uint32_t value) {
   int n = 0;
   do {
@@ -394,6 +376,18 @@ __device__ __forceinline__ int write_varint_bytes(uint8_t* out, uint32_t value) 
 // Residuals of one lane: eight consecutive symbols per 16-byte quad (u16)
 // or two quads (u32), as a native vector so they stay in registers.
 using q4u = uint32_t __attribute__((ext_vector_type(4)));
+using p16 = uint16_t __attribute__((ext_vector_type(2)));
+
+// Histogram bins: every value from kRansAlphabet on shares the last bin (a
+// shard holding one has no order-0 table).  Symbol b of a chunk has class
+// b % 4; classes 2p and 2p + 1 count in the low and high halves of the
+// words of pair p's rows.
+constexpr uint32_t kHistLast = kRansAlphabet;  // 511
+
+__device__ __forceinline__ void hist_pair(uint32_t* row, uint32_t lo, uint32_t hi) {
+  atomicAdd(&row[lo], 1u);
+  atomicAdd(&row[hi], 0x10000u);
+}
 
 template <typename FT>
 struct Chunk8;
@@ -419,6 +413,40 @@ struct Chunk8<uint16_t> {
   __device__ __forceinline__ uint32_t get(int b) const {
     const uint32_t w = b < 2 ? v.x : b < 4 ? v.y : b < 6 ? v.z : v.w;
     return (b & 1) ? (w >> 16) : (w & 0xffffu);
+  }
+  static __device__ __forceinline__ p16 pk(uint32_t w) { return __builtin_bit_cast(p16, w); }
+  static __device__ __forceinline__ uint32_t un(p16 w) { return __builtin_bit_cast(uint32_t, w); }
+  __device__ __forceinline__ void hist_add(uint32_t* row0, uint32_t* row1) const {
+    const p16 last = {(uint16_t)kHistLast, (uint16_t)kHistLast};
+    const uint32_t w0 = un(__builtin_elementwise_min(pk(v.x), last));
+    const uint32_t w1 = un(__builtin_elementwise_min(pk(v.y), last));
+    const uint32_t w2 = un(__builtin_elementwise_min(pk(v.z), last));
+    const uint32_t w3 = un(__builtin_elementwise_min(pk(v.w), last));
+    hist_pair(row0, w0 & 0xffffu, w0 >> 16);
+    hist_pair(row1, w1 & 0xffffu, w1 >> 16);
+    hist_pair(row0, w2 & 0xffffu, w2 >> 16);
+    hist_pair(row1, w3 & 0xffffu, w3 >> 16);
+  }
+  __device__ __forceinline__ uint32_t max8() const {
+    const uint32_t m = un(__builtin_elementwise_max(__builtin_elementwise_max(pk(v.x), pk(v.y)),
+                                                    __builtin_elementwise_max(pk(v.z), pk(v.w))));
+    return max(m & 0xffffu, m >> 16);
+  }
+  __device__ __forceinline__ uint32_t or8() const {
+    const uint32_t o = v.x | v.y | v.z | v.w;
+    return (o | (o >> 16)) & 0xffffu;
+  }
+  // bit b set where symbol b is zero
+  __device__ __forceinline__ uint32_t zero_bits() const {
+    const p16 one = {1, 1};
+    const uint32_t t0 = un(__builtin_elementwise_min(pk(v.x), one));
+    const uint32_t t1 = un(__builtin_elementwise_min(pk(v.y), one));
+    const uint32_t t2 = un(__builtin_elementwise_min(pk(v.z), one));
+    const uint32_t t3 = un(__builtin_elementwise_min(pk(v.w), one));
+    // nonzero flags of symbols 0, 2, 4, 6 at bits 0, 2, 4, 6 and of 1, 3,
+    // 5, 7 at bits 16, 18, 20, 22
+    const uint32_t nz = t0 | (t1 << 2) | (t2 << 4) | (t3 << 6);
+    return ~(nz | (nz >> 15)) & 0xffu;
   }
 };
 
@@ -446,35 +474,92 @@ struct Chunk8<uint32_t> {
       case 4: return c.x; case 5: return c.y; case 6: return c.z; default: return c.w;
     }
   }
+  __device__ __forceinline__ void hist_add(uint32_t* row0, uint32_t* row1) const {
+    hist_pair(row0, min(a.x, kHistLast), min(a.y, kHistLast));
+    hist_pair(row1, min(a.z, kHistLast), min(a.w, kHistLast));
+    hist_pair(row0, min(c.x, kHistLast), min(c.y, kHistLast));
+    hist_pair(row1, min(c.z, kHistLast), min(c.w, kHistLast));
+  }
+  __device__ __forceinline__ uint32_t max8() const {
+    return max(max(max(a.x, a.y), max(a.z, a.w)), max(max(c.x, c.y), max(c.z, c.w)));
+  }
+  __device__ __forceinline__ uint32_t or8() const { return a.x | a.y | a.z | a.w | c.x | c.y | c.z | c.w; }
+  __device__ __forceinline__ uint32_t zero_bits() const {
+    uint32_t z = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) z |= (uint32_t)(get(b) == 0) << b;
+    return z;
+  }
 };
 
-// One wave64 per shard.  Lane l holds symbols 512 c + 8 l + b (chunk c < 8,
-// b < 8), loaded with eight 16-byte loads issued together (every tile's
-// residuals start 16-byte aligned and are padded to whole quads), so symbol i
-// has Rice lane i % 4 = b % 4 and a 128-symbol block is one 16-lane row of
-// one chunk.  Everything the reference computes with shared atomics, a tree
-// reduction and one serial thread (encode.cu:195-431) becomes per-lane
-// counts, ballots, popcounts and wave scans here.
+// Inclusive composition scan of clamp functions over the 64 lanes (lane 0
+// first): row shifts then row broadcasts, lanes without a source composing
+// with the identity.
+__device__ __forceinline__ ClampFn clamp_scan(ClampFn g) {
+  const ClampFn id = clamp_identity();
+  auto step = [&](auto mov) __attribute__((always_inline)) {
+    ClampFn o;
+    o.d = (int)mov((uint32_t)id.d, (uint32_t)g.d);
+    o.lo = (int)mov((uint32_t)id.lo, (uint32_t)g.lo);
+    o.hi = (int)mov((uint32_t)id.hi, (uint32_t)g.hi);
+    g = clamp_compose(o, g);
+  };
+  step([](uint32_t o, uint32_t v) { return dpp_mov<kDppRowShr + 1, 0xf, false>(o, v); });
+  step([](uint32_t o, uint32_t v) { return dpp_mov<kDppRowShr + 2, 0xf, false>(o, v); });
+  step([](uint32_t o, uint32_t v) { return dpp_mov<kDppRowShr + 4, 0xf, false>(o, v); });
+  step([](uint32_t o, uint32_t v) { return dpp_mov<kDppRowShr + 8, 0xf, false>(o, v); });
+  step([](uint32_t o, uint32_t v) { return dpp_mov<kDppBcast15, 0xa, false>(o, v); });
+  step([](uint32_t o, uint32_t v) { return dpp_mov<kDppBcast31, 0xc, false>(o, v); });
+  return g;
+}
+
+// Per-wave LDS of k_analyze.  Histogram rows are kHistRow words apart, so
+// the two copies of a value fall sixteen banks apart; lane l counts into
+// copy l % kHistCopies, which halves the atomics that meet on one address
+// (the residuals of a shard crowd into a few small values).  The list of
+// present values by rank reuses the histogram once its bins are read.
+constexpr int kHistCopies = 2;
+constexpr int kHistRow = 512 + 16;
+
+struct AnalyzeLds {
+  union {
+    uint32_t hist[2][kHistCopies][kHistRow];  // [class pair][copy][bin]
+    uint4 list[512];  // by rank: {value, class pair 0 counts, class pair 1 counts, count}
+  };
+  uint8_t rank_of[512];  // value -> rank
+  uint64_t zw[64];       // zero bits of symbols [64 j, 64 j + 64)
+};
+
+// One wave64 per shard (one wave per block).  Lane l holds symbols
+// 512 c + 8 l + b (chunk c < 8, b < 8), loaded with eight 16-byte loads
+// issued together (every tile's residuals start 16-byte aligned and are
+// padded to whole quads), so symbol i has Rice lane i % 4 = b % 4 and a
+// 128-symbol block is one 16-lane row of one chunk.  Everything the
+// reference computes with shared atomics, a tree reduction and one serial
+// thread (encode.cu:195-431) becomes per-lane counts, class-split histogram
+// atomics, ballots, popcounts and DPP scans here.  The order-0 table works
+// on the compacted list of present values (rank i on lane i % 64), and the
+// Rice and zero-run costs come from the same list, so only the histogram,
+// the zero masks, the block ORs and the rank bytes touch every symbol.  A
+// shard with a value past the histogram (no order-0 table) or more than 511
+// distinct values falls back to per-symbol Rice sums.
 template <typename FT>
-__global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ frames,
-                                                 FrameState* __restrict__ states,
-                                                 const TileDesc* __restrict__ tiles,
-                                                 const ShardDesc* __restrict__ shards,
-                                                 int shard_total, const FT* __restrict__ folded,
-                                                 ShardRec* __restrict__ recs,
-                                                 uint8_t* __restrict__ scratch,
-                                                 uint8_t* __restrict__ rank_area) {
-  __shared__ uint32_t hist_all[4][512];
-  __shared__ uint64_t zw_all[4][64];
+__global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ frames,
+                                                FrameState* __restrict__ states,
+                                                const TileDesc* __restrict__ tiles,
+                                                const ShardDesc* __restrict__ shards,
+                                                int shard_total, const FT* __restrict__ folded,
+                                                ShardRec* __restrict__ recs,
+                                                uint8_t* __restrict__ scratch,
+                                                uint8_t* __restrict__ rank_area) {
+  __shared__ AnalyzeLds lds;
   const int lane = lane_id();
-  const int wave = threadIdx.x >> 6;
-  const int s = blockIdx.x * 4 + wave;
+  const int s = blockIdx.x;
   if (s >= shard_total) return;
   const ShardDesc sd = shards[s];
   const TileDesc td = tiles[sd.tile];
   FrameState* st = &states[td.frame];
   if (!st->active) return;
-  uint32_t* hist = hist_all[wave];
   const int force = frames[td.frame].force_order0;
   const int64_t local0 = (int64_t)sd.local * kShardSymbols;
   const int count = (int)min((int64_t)kShardSymbols, (int64_t)td.width * td.height - local0);
@@ -487,209 +572,183 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
     const int base = 512 * c + 8 * lane;
     ch[c].load(v + (base < count ? base : 0));
   }
-  for (int i = lane; i < 512; i += 64) hist[i] = 0;
+  {
+    uint4* h = (uint4*)&lds.hist[0][0][0];
+    constexpr int kQuads = 2 * kHistCopies * kHistRow / 4;
+    for (int i = lane; i < kQuads; i += 64) h[i] = make_uint4(0, 0, 0, 0);
+  }
   // symbols past the shard's end read as zeros; they are taken out of the
-  // zero masks and of the histogram's count of zeros below
+  // zero masks and of the histogram's zero bins below
   const bool full = count == kShardSymbols;
   if (!full) {
 #pragma unroll
     for (int c = 0; c < 8; ++c) ch[c].keep(clampi(count - (512 * c + 8 * lane), 0, 8));
   }
-  __builtin_amdgcn_wave_barrier();
+  __syncthreads();
 
+  uint32_t* const row0 = lds.hist[0][lane % kHistCopies];
+  uint32_t* const row1 = lds.hist[1][lane % kHistCopies];
   uint32_t lmax = 0;
-  int unsup = 0;
-  int zeros = 0, starts = 0, over64 = 0, over8192 = 0;
+  int zeros = 0, starts = 0;
   uint32_t zmask[8];  // zero bits of each chunk's eight symbols
-  uint32_t prev_last = 0;  // symbol before this lane's chunk is zero
+  uint32_t orv[8];    // OR of each chunk's eight symbols
+  uint32_t prev_last = 0;  // symbol before lane 0's chunk is zero
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    uint32_t zm = 0;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint32_t x = ch[c].get(b);
-      lmax = max(lmax, x);
-      zm |= (uint32_t)(x == 0) << b;
-      over64 += x > 64;
-      over8192 += x > 8192;
-      if (x < (uint32_t)kRansAlphabet) atomicAdd(&hist[x], 1u);
-      else unsup = 1;
-    }
+    ch[c].hist_add(row0, row1);
+    lmax = max(lmax, ch[c].max8());
+    orv[c] = ch[c].or8();
+    uint32_t zm = ch[c].zero_bits();
     if (!full) zm &= (1u << clampi(count - (512 * c + 8 * lane), 0, 8)) - 1u;
     zmask[c] = zm;
     zeros += __popc(zm);
     // the symbol before this lane's chunk: lane - 1's last, or for lane 0 the
     // previous chunk's last symbol (lane 63)
-    const uint32_t up = shfl_up(zm >> 7, 1);
-    prev_last = lane == 0 ? prev_last : up;
-    starts += __popc(zm & ~((zm << 1) | (prev_last & 1u)));
-    prev_last = shfl(zm >> 7, 63);  // for lane 0 of the next chunk
+    const uint32_t up = from_prev_lane(zm >> 7);
+    const uint32_t before = lane == 0 ? prev_last : up;
+    starts += __popc(zm & ~((zm << 1) | (before & 1u)));
+    prev_last = (uint32_t)__builtin_amdgcn_readlane((int)(zm >> 7), 63);
   }
-  if (!full && lane == 0) atomicSub(&hist[0], (uint32_t)(kShardSymbols - count));
-#pragma unroll
-  for (int c = 0; c < 8; ++c) ch[c].launder();
-  __builtin_amdgcn_wave_barrier();
+  __syncthreads();
   const uint32_t maxv = wave_max(lmax);
   zeros = wave_sum(zeros);
   starts = wave_sum(starts);
-  over64 = wave_sum(over64);
-  over8192 = wave_sum(over8192);
-  bool supported = ballot(unsup != 0) == 0;
+  const bool exact = maxv < kHistLast;  // every value in a bin of its own
 
-  ShardRec rec = {};
-  rec.best3_body = 0xffffffffu;
+  // record fields, kept as scalars until the one store at the end
+  uint32_t r_mode = kModeOrder0, r_k = 0, r_best3_mode = 0, r_sparse_len = 0;
+  uint32_t r_body = 0, r_best3_body = 0xffffffffu;
+  uint32_t r_lane[4] = {0, 0, 0, 0};
   uint8_t* sbase = scratch + (int64_t)s * kScratchBytes;
 
-  // ---------------- order-0 table (encode.cu:337-392)
+  // ---------------- present values by rank
+  // bins 8 lane + k, copies summed; the padding zeros leave class r's zero
+  // bin (1024 - (count - r + 3) / 4 of them)
   int distinct = 0;
-  if (supported) {
-    uint32_t c[8];
-    uint32_t csum = 0;
-    int npres = 0, last = -1;
+  if (exact) {
+    uint32_t c01[8], c23[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      c[k] = hist[lane * 8 + k];
-      csum += c[k];
-      if (c[k]) {
-        ++npres;
-        last = lane * 8 + k;
+    for (int p = 0; p < 2; ++p) {
+      uint32_t* acc = p == 0 ? c01 : c23;
+#pragma unroll
+      for (int cp = 0; cp < kHistCopies; ++cp) {
+        const uint4 q0 = *(const uint4*)&lds.hist[p][cp][8 * lane];
+        const uint4 q1 = *(const uint4*)&lds.hist[p][cp][8 * lane + 4];
+        const uint32_t w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] = cp == 0 ? w[k] : acc[k] + w[k];
       }
     }
-    const int rank_incl = wave_incl_scan(npres);
-    const uint32_t cnt_incl = wave_incl_scan(csum);
-    distinct = shfl(rank_incl, 63);
-    if (distinct > 255) supported = false;
-    if (supported) {
-      const int rank0 = rank_incl - npres;
-      // compose this lane's normalisation steps
-      ClampFn g = clamp_identity();
-      {
-        uint32_t p = cnt_incl - csum;
-        int rank = rank0;
+    if (!full && lane == 0) {
+      const uint32_t pad0 = 1024u - (uint32_t)((count + 3) >> 2);
+      const uint32_t pad1 = 1024u - (uint32_t)((count + 2) >> 2);
+      const uint32_t pad2 = 1024u - (uint32_t)((count + 1) >> 2);
+      const uint32_t pad3 = 1024u - (uint32_t)(count >> 2);
+      c01[0] -= pad0 | (pad1 << 16);
+      c23[0] -= pad2 | (pad3 << 16);
+    }
+    uint32_t cnt[8];
+    uint32_t pm = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          if (c[k]) {
-            p += c[k];
-            g = clamp_compose(g, norm_fn(p, count, distinct - rank - 1));
-            ++rank;
-          }
-        }
+    for (int k = 0; k < 8; ++k) {
+      cnt[k] = (c01[k] & 0xffffu) + (c01[k] >> 16) + (c23[k] & 0xffffu) + (c23[k] >> 16);
+      pm |= (uint32_t)(cnt[k] != 0) << k;
+    }
+    const int npres = __popc(pm);
+    const int incl = wave_incl_scan(npres);
+    distinct = __builtin_amdgcn_readlane(incl, 63);
+    const int rank0 = incl - npres;
+    __syncthreads();  // the list overwrites the bins just read (one wave per block)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if ((pm >> k) & 1u) {
+        lds.list[rank0 + __popc(pm & ((1u << k) - 1u))] =
+            make_uint4((uint32_t)(8 * lane + k), c01[k], c23[k], cnt[k]);
       }
-#pragma unroll
-      for (int dlt = 1; dlt < 64; dlt <<= 1) {
-        ClampFn o;
-        o.d = shfl_up(g.d, dlt);
-        o.lo = shfl_up(g.lo, dlt);
-        o.hi = shfl_up(g.hi, dlt);
-        if (lane >= dlt) g = clamp_compose(o, g);
+    }
+    __syncthreads();
+  }
+  const bool supported = exact && distinct <= 255;
+  const int blocks = (distinct + 63) >> 6;  // 64-rank blocks of the list
+
+  // ---------------- order-0 table (encode.cu:337-392): frequencies by
+  // the clamp-function scan of the normalisation, encoder entries by rank,
+  // value -> rank map, sparse table (varint value gap, varint frequency
+  // except for the last symbol)
+  if (supported) {
+    uint2* tab = (uint2*)(sbase + kScratchTable);
+    uint8_t* sparse = sbase + kScratchSparse;
+    const int hdr = 1 + varint_len((uint32_t)distinct);
+    if (lane == 0) {
+      sparse[0] = (uint8_t)kRansTableLog;
+      write_varint_bytes(sparse + 1, (uint32_t)distinct);
+    }
+    int carry_end = 0, carry_pos = hdr;
+    uint32_t carry_p = 0, carry_v = 0;
+    for (int t = 0; t < blocks; ++t) {
+      const int i = 64 * t + lane;
+      const bool valid = i < distinct;
+      const uint4 e = valid ? lds.list[i] : make_uint4(0, 0, 0, 0);
+      const uint32_t p = carry_p + wave_incl_scan(e.w);
+      const int rem = distinct - i - 1;
+      const ClampFn g = clamp_scan(valid ? norm_fn(p, count, rem) : clamp_identity());
+      const int end = clamp_apply(g, carry_end);
+      const uint32_t prev_end = from_prev_lane((uint32_t)end);
+      const int cum = lane == 0 ? carry_end : (int)prev_end;
+      const uint32_t freq = (uint32_t)(end - cum);
+      const uint32_t prev_v = from_prev_lane(e.x);
+      const uint32_t gap = e.x - (lane == 0 ? carry_v : prev_v);
+      const int len = valid ? varint_len(gap) + (rem ? varint_len(freq) : 0) : 0;
+      const int len_incl = wave_incl_scan(len);
+      if (valid) {
+        tab[i] = make_uint2(freq | ((uint32_t)cum << 13), 0u);
+        lds.rank_of[e.x] = (uint8_t)i;
+        const int pos = carry_pos + len_incl - len;
+        const int n = write_varint_bytes(sparse + pos, gap);
+        if (rem) write_varint_bytes(sparse + pos + n, freq);
       }
-      ClampFn e;
-      e.d = shfl_up(g.d, 1);
-      e.lo = shfl_up(g.lo, 1);
-      e.hi = shfl_up(g.hi, 1);
-      if (lane == 0) e = clamp_identity();
-      int assigned = clamp_apply(e, 0);
-      // previous present value before this lane (0 if none)
-      int prev_val = last;
+      carry_end = __builtin_amdgcn_readlane(end, 63);
+      carry_p = (uint32_t)__builtin_amdgcn_readlane((int)p, 63);
+      carry_v = (uint32_t)__builtin_amdgcn_readlane((int)e.x, 63);
+      carry_pos += __builtin_amdgcn_readlane(len_incl, 63);
+    }
+    r_sparse_len = (uint32_t)carry_pos;
+    // rank bytes of rANS lane r (symbols 4k + r): the rank of symbol 4k + r
+    // at byte p = 1023 - k, in the group-interleaved chunks of the rank
+    // area (the rANS lanes visit symbols from the last one back, so each
+    // reads its bytes forwards).  Symbols b = r and b = r + 4 of a chunk
+    // are k and k + 1 of lane r: one 16-bit store at byte 1022 - k, which
+    // is even, so both bytes fall in one 16-byte chunk.
+    __syncthreads();
 #pragma unroll
-      for (int dlt = 1; dlt < 64; dlt <<= 1) {
-        const int o = shfl_up(prev_val, dlt);
-        if (lane >= dlt) prev_val = max(prev_val, o);
-      }
-      int before = shfl_up(prev_val, 1);
-      if (lane == 0 || before < 0) before = 0;
-      // frequencies, encoder entries (by rank), value -> rank map in the
-      // histogram's LDS, sparse-table entry sizes
-      uint2* tab = (uint2*)(sbase + kScratchTable);
-      uint32_t freqs[8];
-      int lens = 0;
-      {
-        uint32_t p = cnt_incl - csum;
-        int rank = rank0;
-        int pv = before;
+    for (int c = 0; c < 8; ++c) ch[c].launder();
+    uint8_t* ranks = rank_area + (int64_t)(s / kRansShards) * kRankGroupBytes +
+                     16 * 4 * (s % kRansShards);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          freqs[k] = 0;
-          if (c[k]) {
-            p += c[k];
-            const int rem = distinct - rank - 1;
-            const int end = clamp_apply(norm_fn(p, count, rem), assigned);
-            freqs[k] = (uint32_t)(end - assigned);
-            tab[rank] = rans_entry(freqs[k], (uint32_t)assigned);
-            hist[lane * 8 + k] = (uint32_t)rank;
-            lens += varint_len((uint32_t)(lane * 8 + k - pv)) + (rem ? varint_len(freqs[k]) : 0);
-            assigned = end;
-            pv = lane * 8 + k;
-            ++rank;
-          }
-        }
-      }
-      const int len_incl = wave_incl_scan(lens);
-      const int hdr = 1 + varint_len((uint32_t)distinct);
-      uint8_t* sparse = sbase + kScratchSparse;
-      if (lane == 0) {
-        sparse[0] = (uint8_t)kRansTableLog;
-        write_varint_bytes(sparse + 1, (uint32_t)distinct);
-      }
-      {
-        int pos = hdr + len_incl - lens;
-        int rank = rank0;
-        int pv = before;
+    for (int c = 0; c < 8; ++c) {
+      const int p = 1022 - (128 * c + 2 * lane);  // byte of symbol b = r + 4 (lane r)
+      uint8_t* const at = ranks + 1024 * (p >> 4) + (p & 15);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          if (c[k]) {
-            const int rem = distinct - rank - 1;
-            pos += write_varint_bytes(sparse + pos, (uint32_t)(lane * 8 + k - pv));
-            if (rem) pos += write_varint_bytes(sparse + pos, freqs[k]);
-            pv = lane * 8 + k;
-            ++rank;
-          }
-        }
-      }
-      rec.sparse_len = (uint16_t)(hdr + shfl(len_incl, 63));
-      rec.distinct = (uint16_t)distinct;
-      rec.supported = 1;
-      // rank bytes of rANS lane r (symbols 4k + r): the rank of symbol 4k + r
-      // at byte p = 1023 - k, in the group-interleaved chunks of the rank
-      // area (the rANS lanes visit symbols from the last one back, so each
-      // reads its bytes forwards).  Symbols b = r and b = r + 4 of a chunk
-      // are k and k + 1 of lane r: one 16-bit store at byte 1022 - k, which
-      // is even, so both bytes fall in one 16-byte chunk.
-      __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int c = 0; c < 8; ++c) ch[c].launder();
-      uint8_t* ranks = rank_area + (int64_t)(s / kRansShards) * kRankGroupBytes +
-                       16 * 4 * (s % kRansShards);
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int p = 1022 - (128 * c + 2 * lane);  // byte of symbol b = r + 4 (lane r)
-        uint8_t* const at = ranks + 1024 * (p >> 4) + (p & 15);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const uint32_t rk0 = hist[ch[c].get(r)] & 0xffu;
-          const uint32_t rk1 = hist[ch[c].get(r + 4)] & 0xffu;
-          *(uint16_t*)(at + 16 * r) = (uint16_t)(rk1 | (rk0 << 8));
-        }
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t rk0 = lds.rank_of[ch[c].get(r)];
+        const uint32_t rk1 = lds.rank_of[ch[c].get(r + 4)];
+        *(uint16_t*)(at + 16 * r) = (uint16_t)(rk1 | (rk0 << 8));
       }
     }
   }
 
   // ---------------- best of zero-run / Rice / block-pack (encode.cu:226-307)
   if (!force || !supported) {
-#pragma unroll
-    for (int c = 0; c < 8; ++c) ch[c].launder();
     // long zero runs (> 64 symbols cost a 2-byte token): runs whose start p
     // has zeros at p..p+64, found from the 64-bit zero word of symbols
     // [64 lane, 64 lane + 64) and the next word
-    uint8_t* zb = (uint8_t*)zw_all[wave];
+    uint8_t* zb = (uint8_t*)lds.zw;
 #pragma unroll
     for (int c = 0; c < 8; ++c) zb[64 * c + lane] = (uint8_t)zmask[c];
-    __builtin_amdgcn_wave_barrier();
-    const uint64_t myZ = zw_all[wave][lane];
-    uint64_t zp = shfl_up(myZ, 1);
-    uint64_t zn = __shfl_down(myZ, 1, 64);
-    if (lane == 0) zp = 0;
-    if (lane == 63) zn = 0;
+    __syncthreads();
+    const uint64_t myZ = lds.zw[lane];
+    const uint64_t zp = from_prev_lane(myZ);
+    const uint64_t zn = from_next_lane(myZ);
     const uint64_t st_bits = myZ & ~((myZ << 1) | (zp >> 63));
     const uint64_t nz = ~myZ, nzn = ~zn;
     const int lo = nz == 0 ? 0 : 64 - __clzll((long long)nz);
@@ -700,25 +759,45 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
       m = upper & ~((1ull << lo) - 1);
     }
     const int longs = wave_sum(popc64(st_bits & m));
+    // values past 64 and past 8192 take longer tokens: from the list, or
+    // symbol by symbol when values fall past the histogram
+    int over64 = 0, over8192 = 0;
+    if (exact) {
+      for (int t = 0; t < blocks; ++t) {
+        const int i = 64 * t + lane;
+        if (i < distinct) {
+          const uint4 e = lds.list[i];
+          over64 += e.x > 64 ? (int)e.w : 0;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const uint32_t x = ch[c].get(b);
+          over64 += x > 64;
+          over8192 += x > 8192;
+        }
+      }
+      over8192 = wave_sum(over8192);
+    }
+    over64 = wave_sum(over64);
     const int64_t zero_bytes = (int64_t)(count - zeros) + over64 + over8192 + starts + longs;
 
     // block b = chunk b / 4, row b % 4 of the wave: OR over the row
     int block_part = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      uint32_t o = 0;
-#pragma unroll
-      for (int b = 0; b < 8; ++b) o |= ch[c].get(b);
-      o |= shfl_xor(o, 1);
-      o |= shfl_xor(o, 2);
-      o |= shfl_xor(o, 4);
-      o |= shfl_xor(o, 8);
+      const uint32_t o = row_or(orv[c]);
       const int blk = 4 * c + (lane >> 4);
       const int n = clampi(count - blk * kBlockSymbols, 0, kBlockSymbols);
       if ((lane & 15) == 0 && n > 0) block_part += 1 + (n * bit_length(o) + 7) / 8;
     }
     const int64_t block_bytes = wave_sum(block_part);
 
+    // Rice: class r (symbol index mod 4) sums of x >> k, from the list's
+    // class counts, or symbol by symbol
     const int lane_count = count < 4 ? count : 4;
     const int bl = bit_length(maxv);
     int best_k = 0;
@@ -726,13 +805,26 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
     uint32_t best_lane[4] = {0, 0, 0, 0};
 #pragma unroll 1
     for (int k = 0; k <= kMaxRice; ++k) {
-      // class r = symbol index mod 4 = b mod 4
       uint32_t cs[4] = {0, 0, 0, 0};
       if (k < bl) {
+        if (exact) {
+          for (int t = 0; t < blocks; ++t) {
+            const int i = 64 * t + lane;
+            if (i < distinct) {
+              const uint4 e = lds.list[i];
+              const uint32_t q = e.x >> k;
+              cs[0] += __umul24(e.y & 0xffffu, q);
+              cs[1] += __umul24(e.y >> 16, q);
+              cs[2] += __umul24(e.z & 0xffffu, q);
+              cs[3] += __umul24(e.z >> 16, q);
+            }
+          }
+        } else {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
+          for (int c = 0; c < 8; ++c) {
 #pragma unroll
-          for (int b = 0; b < 8; ++b) cs[b & 3] += ch[c].get(b) >> k;
+            for (int b = 0; b < 8; ++b) cs[b & 3] += ch[c].get(b) >> k;
+          }
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) cs[r] = wave_sum(cs[r]);
@@ -768,20 +860,33 @@ __global__ __launch_bounds__(256) void k_analyze(const FrameDesc* __restrict__ f
       body = rice_bytes;
     }
     if (body > 65535 && lane == 0) set_status(st, kStatusBodyTooLarge);
-    rec.best3_mode = (uint8_t)mode;
-    rec.mode = (uint8_t)mode;
-    rec.k = (uint8_t)best_k;
-    rec.best3_body = (uint32_t)min(body, (int64_t)0xffffffff);
-    rec.body = rec.best3_body;
+    r_best3_mode = (uint32_t)mode;
+    r_mode = (uint32_t)mode;
+    r_k = (uint32_t)best_k;
+    r_best3_body = (uint32_t)min(body, (int64_t)0xffffffff);
+    r_body = r_best3_body;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) r_lane[r] = min(best_lane[r], 65535u);
+  }
+  if (lane == 0) {
+    ShardRec rec;
+    rec.mode = (uint8_t)r_mode;
+    rec.k = (uint8_t)r_k;
+    rec.supported = (uint8_t)supported;
+    rec.best3_mode = (uint8_t)r_best3_mode;
+    rec.sparse_len = (uint16_t)r_sparse_len;
+    rec.distinct = (uint16_t)(supported ? distinct : 0);
+    rec.body = r_body;
+    rec.best3_body = r_best3_body;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      rec.best3_lane[r] = (uint16_t)min(best_lane[r], 65535u);
-      rec.lane[r] = rec.best3_lane[r];
+      rec.lane[r] = (uint16_t)r_lane[r];
+      rec.best3_lane[r] = (uint16_t)r_lane[r];
+      rec.states[r] = 0;
     }
-  } else {
-    rec.mode = kModeOrder0;
+    rec.offset = 0;
+    recs[s] = rec;
   }
-  if (lane == 0) recs[s] = rec;
 }
 
 // Order-0 rANS encode (encode.cu:398-418): one thread per rANS lane, four
@@ -1401,7 +1506,7 @@ static void encode_pass(const EncodeBuffers& b, int pass, hipStream_t stream) {
   }
   {
     FVH_TIMED("k_analyze", stream);
-    hipLaunchKernelGGL((k_analyze<FT>), dim3((b.shard_total + 3) / 4), dim3(256), 0, stream,
+    hipLaunchKernelGGL((k_analyze<FT>), dim3(b.shard_total), dim3(64), 0, stream,
                        b.frames, b.states, b.tiles, b.shards, b.shard_total, folded, b.recs,
                        b.scratch, b.ranks);
   }
