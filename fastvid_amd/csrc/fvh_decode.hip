@@ -669,9 +669,8 @@ __global__ __launch_bounds__(256) void k_o0_parse(const FrameDesc* __restrict__ 
 // where a symbol's slot range starts at slot 32 b + j (the first symbol
 // excluded) and prefix[b] counts the starts before the bucket, so the rank of
 // a slot is prefix[b] + popcount(mask[b] & ((2 << j) - 1)).  Rank -> symbol:
-// ent[k] = cum_k | (freq_k - 4096 mod 2^13) << 13 and, at a fixed distance
-// in the same row, the value, read together by one two-dword LDS read.  The
-// state update freq*(x >> 12) + (x & 4095) - cum is computed as
+// the pair {cum_k | (freq_k - 4096 mod 2^13) << 13, value_k}, one 8-byte LDS
+// read.  The state update freq*(x >> 12) + (x & 4095) - cum is computed as
 // (freq - 4096)*(x >> 12) + (x - cum), one signed 24-bit multiply-add whose
 // low 32 bits equal the reference's (exact) u32 result.  Tables: 640 bytes +
 // 8 per symbol, against the reference's 4096-byte slot table
@@ -721,15 +720,13 @@ __device__ __forceinline__ uint32_t bcnt_add(uint32_t a, uint32_t b) {
   return r;
 }
 
-// Entry rows: kMax entries, then the values at a distance the two-dword LDS
-// read can express (up to 255 dwords, or a multiple of 64).  Row and stage
-// strides are odd in dwords: the sixteen shards of a wave mostly read the same
-// few ranks, and an even stride would put them on a few banks.
+// Entry rows: kMax {entry, value} pairs, rounded to a multiple of 16 pairs,
+// plus one: the sixteen shards of a wave mostly read their first few ranks,
+// and with a row stride of 2 mod 32 dwords their pairs cover the 32 banks
+// once.  The stage stride is odd in dwords for the same reason.
 template <int kMax>
 struct O0Row {
-  static constexpr int kEnt = (kMax + 3) & ~3;
-  static constexpr int kVal = kEnt <= 128 ? kEnt : 256;
-  static constexpr int kWords = (kVal + kEnt) | 1;
+  static constexpr int kPairs = ((kMax + 15) & ~15) + 1;
 };
 
 template <typename FT>
@@ -740,7 +737,7 @@ struct O0Stage {
 template <typename FT, int kMax>
 struct O0Lds {
   uint32_t mask[kO0Shards][kO0Buckets];
-  uint32_t row[kO0Shards][O0Row<kMax>::kWords];
+  uint2 row[kO0Shards][O0Row<kMax>::kPairs];
   uint8_t prefix[kO0Shards][kO0Buckets];
   uint32_t stage[kO0Shards][O0Stage<FT>::kWords];
 };
@@ -756,14 +753,13 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
                                                FT* __restrict__ folded, char* gbase,
                                                uint8_t* __restrict__ sink,
                                                int32_t* __restrict__ queue) {
-  using Row = O0Row<kMax>;
   __shared__ O0Lds<FT, kMax> lds;
   const int lane = lane_id();
   const int g = lane >> 2;
   const int r = lane & 3;
   const int count = *tier_count;
   uint32_t* const mask = lds.mask[g];
-  uint32_t* const ent = lds.row[g];
+  uint2* const ent = lds.row[g];
   uint8_t* const prefix = lds.prefix[g];
   FT* const stage = (FT*)lds.stage[g];
   // a wave's first group is its block index, later ones come from the
@@ -783,8 +779,7 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
       const uint32_t e0 = tab->ent[k];
       const uint32_t c = e0 & 0x1fffu;
       const uint32_t fr = (tab->ent[k + 1] & 0x1fffu) - c;
-      ent[k] = c | (((fr - 4096u) & 0x1fffu) << 13);
-      ent[Row::kVal + k] = e0 >> 13;
+      ent[k] = make_uint2(c | (((fr - 4096u) & 0x1fffu) << 13), e0 >> 13);
       if (k > 0) atomicOr(&mask[c >> 5], 1u << (c & 31u));
     }
     __builtin_amdgcn_wave_barrier();
@@ -834,7 +829,11 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
     const Chunk A0 = load_chunk(cp);
     const Chunk B0 = load_chunk(cp + min(1, last));
     const Chunk C0 = load_chunk(cp + min(2, last));
+    const Chunk D0 = load_chunk(cp + min(3, last));
     uint64_t a0 = A0.lo, a1 = A0.hi, b0 = B0.lo, b1 = B0.hi, c0 = C0.lo, c1 = C0.hi;
+    uint64_t d0 = D0.lo, d1 = D0.hi;  // chunk ci + 3, or in flight into n0:n1
+    uint64_t n0 = 0, n1 = 0;
+    bool pend = false;   // n0:n1 holds chunk ci + 3 once its load lands
     int ci = 0;          // chunk index of A
     int pos = head;      // next byte within A:B, < 16 when an iteration starts
     int consumed = 0;    // lane bits consumed
@@ -858,14 +857,16 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
       const uint32_t b = (x >> 5) & (kO0Buckets - 1);
       const uint32_t m = mask[b] & ((2u << (x & 31u)) - 1u);
       const uint32_t rank = bcnt_add(m, prefix[b]);
-      const uint32_t e = ent[rank];
-      const uint32_t v = ent[Row::kVal + rank];
-      stage[4 * slot_pos + r] = (FT)v;
-      const uint32_t c = e & 0x1fffu;
-      const int fneg = ((int)(e << 6)) >> 19;  // freq - 4096, sign-extended
+      const uint2 ev = ent[rank];
+      stage[4 * slot_pos + r] = (FT)ev.y;
+      const uint32_t c = ev.x & 0x1fffu;
+      const int fneg = ((int)(ev.x << 6)) >> 19;  // freq - 4096, sign-extended
       const uint32_t xd = (uint32_t)__mul24(fneg, (int)(x >> kRansTableLog)) + (x - c);
-      // one byte below 2^23, two below 2^15 (never more: xd >= 2^11)
-      int sh = (xd < kRansL ? 8 : 0) + (xd < (1u << 15) ? 8 : 0);
+      // one byte below 2^23, two below 2^15 (never more: xd >= 2^11): the
+      // leading zeros less one, rounded down to a multiple of 8 (none from
+      // 2^31, where a state read from a malformed stream may start)
+      const uint32_t lz = (uint32_t)__builtin_clz(xd);
+      int sh = (int)(__builtin_elementwise_sub_sat(lz, 1u) & 0x18u);
       if (!live) sh = 0;
       const uint64_t wide = (((uint64_t)xd << 32) | (uint32_t)(bb >> 32)) << sh;
       x = live ? (uint32_t)(wide >> 32) : x;
@@ -891,9 +892,10 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
       }
     };
     // Two rounds: the first reads A:B (pos < 16), the second A:B:C (pos < 24).
-    // After them the window moves by one chunk at most, and the chunk after
-    // the new window loads before the iteration's store, so waiting for it
-    // next iteration never waits for that store.
+    // After them the window moves by one chunk at most (C <- D); a lane that
+    // moved loads the chunk after D into n0:n1 as the iteration's last memory
+    // operation (`refill`), and the next iteration takes it into D at its
+    // end, so each load has a whole iteration to land.
     auto iteration = [&](int s0, bool predicated) __attribute__((always_inline)) {
       int used_total = 0;
       {
@@ -915,16 +917,27 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
       consumed += used_total;
       pos += used_total >> 3;
       const bool adv = pos >= 16;
+      // (kept after the rounds: the wait for last iteration's load goes here)
+      __builtin_amdgcn_sched_barrier(0);
+      d0 = pend ? n0 : d0;
+      d1 = pend ? n1 : d1;
       a0 = adv ? b0 : a0;
       a1 = adv ? b1 : a1;
       b0 = adv ? c0 : b0;
       b1 = adv ? c1 : b1;
+      c0 = adv ? d0 : c0;
+      c1 = adv ? d1 : c1;
       pos -= adv ? 16 : 0;
       ci += adv ? 1 : 0;
+      pend = adv;
+    };
+    // (lanes that did not move read one line every such lane of the wave
+    // shares, so the load stays unconditional and n0:n1 needs no merge)
+    auto refill = [&]() __attribute__((always_inline)) {
       __builtin_amdgcn_sched_barrier(0);
-      const Chunk Cn = load_chunk(cp + min(ci + 2, last));
-      c0 = Cn.lo;
-      c1 = Cn.hi;
+      const Chunk Nn = load_chunk(pend ? cp + min(ci + 3, last) : (const uint4*)tabs);
+      n0 = Nn.lo;
+      n1 = Nn.hi;
       __builtin_amdgcn_sched_barrier(0);
     };
 
@@ -934,10 +947,12 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
       // every lane of the wave has all 32 symbols here (full_steps >= s0 + 8):
       // one unconditional store, so the memory waits stay exact
       flush_full(s0);
+      refill();
     }
     for (; s0 < wave_steps; s0 += 8) {
       iteration(s0, true);
       flush(s0);
+      refill();
     }
     if (nsym > 0 && (consumed != 8 * length || x != kRansL)) set_status(&states[d.frame], 5);
     int next = 0;
