@@ -381,7 +381,9 @@ using p16 = uint16_t __attribute__((ext_vector_type(2)));
 // Histogram bins: every value from kRansAlphabet on shares the last bin (a
 // shard holding one has no order-0 table).  Symbol b of a chunk has class
 // b % 4; classes 2p and 2p + 1 count in the low and high halves of the
-// words of pair p's rows.
+// words of pair p's rows.  Zeros, the commonest value by far, are counted
+// from the zero masks instead: their atomics go to a junk slot past the
+// bins, one of eight by lane, so they no longer all meet on bin 0.
 constexpr uint32_t kHistLast = kRansAlphabet;  // 511
 
 __device__ __forceinline__ void hist_pair(uint32_t* row, uint32_t lo, uint32_t hi) {
@@ -416,16 +418,28 @@ struct Chunk8<uint16_t> {
   }
   static __device__ __forceinline__ p16 pk(uint32_t w) { return __builtin_bit_cast(p16, w); }
   static __device__ __forceinline__ uint32_t un(p16 w) { return __builtin_bit_cast(uint32_t, w); }
-  __device__ __forceinline__ void hist_add(uint32_t* row0, uint32_t* row1) const {
+  // histogram atomics (zeros to the junk slot) and the zero bits: bit b
+  // set where symbol b is zero
+  __device__ __forceinline__ uint32_t hist_add(uint32_t* row0, uint32_t* row1, uint32_t junk) const {
     const p16 last = {(uint16_t)kHistLast, (uint16_t)kHistLast};
-    const uint32_t w0 = un(__builtin_elementwise_min(pk(v.x), last));
-    const uint32_t w1 = un(__builtin_elementwise_min(pk(v.y), last));
-    const uint32_t w2 = un(__builtin_elementwise_min(pk(v.z), last));
-    const uint32_t w3 = un(__builtin_elementwise_min(pk(v.w), last));
+    const p16 one = {1, 1};
+    const p16 jk = pk(junk);
+    const p16 n0 = __builtin_elementwise_min(pk(v.x), one);  // 1 where nonzero
+    const p16 n1 = __builtin_elementwise_min(pk(v.y), one);
+    const p16 n2 = __builtin_elementwise_min(pk(v.z), one);
+    const p16 n3 = __builtin_elementwise_min(pk(v.w), one);
+    const uint32_t w0 = un((one - n0) * jk + __builtin_elementwise_min(pk(v.x), last));
+    const uint32_t w1 = un((one - n1) * jk + __builtin_elementwise_min(pk(v.y), last));
+    const uint32_t w2 = un((one - n2) * jk + __builtin_elementwise_min(pk(v.z), last));
+    const uint32_t w3 = un((one - n3) * jk + __builtin_elementwise_min(pk(v.w), last));
     hist_pair(row0, w0 & 0xffffu, w0 >> 16);
     hist_pair(row1, w1 & 0xffffu, w1 >> 16);
     hist_pair(row0, w2 & 0xffffu, w2 >> 16);
     hist_pair(row1, w3 & 0xffffu, w3 >> 16);
+    // nonzero flags of symbols 0, 2, 4, 6 at bits 0, 2, 4, 6 and of 1, 3,
+    // 5, 7 at bits 16, 18, 20, 22
+    const uint32_t nz = un(n0) | (un(n1) << 2) | (un(n2) << 4) | (un(n3) << 6);
+    return ~(nz | (nz >> 15)) & 0xffu;
   }
   __device__ __forceinline__ uint32_t max8() const {
     const uint32_t m = un(__builtin_elementwise_max(__builtin_elementwise_max(pk(v.x), pk(v.y)),
@@ -435,18 +449,6 @@ struct Chunk8<uint16_t> {
   __device__ __forceinline__ uint32_t or8() const {
     const uint32_t o = v.x | v.y | v.z | v.w;
     return (o | (o >> 16)) & 0xffffu;
-  }
-  // bit b set where symbol b is zero
-  __device__ __forceinline__ uint32_t zero_bits() const {
-    const p16 one = {1, 1};
-    const uint32_t t0 = un(__builtin_elementwise_min(pk(v.x), one));
-    const uint32_t t1 = un(__builtin_elementwise_min(pk(v.y), one));
-    const uint32_t t2 = un(__builtin_elementwise_min(pk(v.z), one));
-    const uint32_t t3 = un(__builtin_elementwise_min(pk(v.w), one));
-    // nonzero flags of symbols 0, 2, 4, 6 at bits 0, 2, 4, 6 and of 1, 3,
-    // 5, 7 at bits 16, 18, 20, 22
-    const uint32_t nz = t0 | (t1 << 2) | (t2 << 4) | (t3 << 6);
-    return ~(nz | (nz >> 15)) & 0xffu;
   }
 };
 
@@ -474,22 +476,22 @@ struct Chunk8<uint32_t> {
       case 4: return c.x; case 5: return c.y; case 6: return c.z; default: return c.w;
     }
   }
-  __device__ __forceinline__ void hist_add(uint32_t* row0, uint32_t* row1) const {
-    hist_pair(row0, min(a.x, kHistLast), min(a.y, kHistLast));
-    hist_pair(row1, min(a.z, kHistLast), min(a.w, kHistLast));
-    hist_pair(row0, min(c.x, kHistLast), min(c.y, kHistLast));
-    hist_pair(row1, min(c.z, kHistLast), min(c.w, kHistLast));
-  }
-  __device__ __forceinline__ uint32_t max8() const {
-    return max(max(max(a.x, a.y), max(a.z, a.w)), max(max(c.x, c.y), max(c.z, c.w)));
-  }
-  __device__ __forceinline__ uint32_t or8() const { return a.x | a.y | a.z | a.w | c.x | c.y | c.z | c.w; }
-  __device__ __forceinline__ uint32_t zero_bits() const {
+  __device__ __forceinline__ uint32_t hist_add(uint32_t* row0, uint32_t* row1, uint32_t junk) const {
+    const uint32_t j = junk & 0xffffu;
+    auto bin = [&](uint32_t x) { return x == 0 ? j : min(x, kHistLast); };
+    hist_pair(row0, bin(a.x), bin(a.y));
+    hist_pair(row1, bin(a.z), bin(a.w));
+    hist_pair(row0, bin(c.x), bin(c.y));
+    hist_pair(row1, bin(c.z), bin(c.w));
     uint32_t z = 0;
 #pragma unroll
     for (int b = 0; b < 8; ++b) z |= (uint32_t)(get(b) == 0) << b;
     return z;
   }
+  __device__ __forceinline__ uint32_t max8() const {
+    return max(max(max(a.x, a.y), max(a.z, a.w)), max(max(c.x, c.y), max(c.z, c.w)));
+  }
+  __device__ __forceinline__ uint32_t or8() const { return a.x | a.y | a.z | a.w | c.x | c.y | c.z | c.w; }
 };
 
 // Inclusive composition scan of clamp functions over the 64 lanes (lane 0
@@ -514,12 +516,14 @@ __device__ __forceinline__ ClampFn clamp_scan(ClampFn g) {
 }
 
 // Per-wave LDS of k_analyze.  Histogram rows are kHistRow words apart, so
-// the two copies of a value fall sixteen banks apart; lane l counts into
-// copy l % kHistCopies, which halves the atomics that meet on one address
-// (the residuals of a shard crowd into a few small values).  The list of
-// present values by rank reuses the histogram once its bins are read.
-constexpr int kHistCopies = 2;
-constexpr int kHistRow = 512 + 16;
+// the four copies of a value fall eight banks apart; lane l counts into
+// copy l % kHistCopies, which quarters the atomics that meet on one address
+// (the residuals of a shard crowd into a few small values; two copies took
+// 17% longer, eight left too few waves per compute unit).  Words 512..519
+// of a row are the zeros' junk slots.  The list of present values by rank
+// reuses the histogram once its bins are read.
+constexpr int kHistCopies = 4;
+constexpr int kHistRow = 512 + 8;
 
 struct AnalyzeLds {
   union {
@@ -588,6 +592,7 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
 
   uint32_t* const row0 = lds.hist[0][lane % kHistCopies];
   uint32_t* const row1 = lds.hist[1][lane % kHistCopies];
+  const uint32_t junk = (512u + ((uint32_t)lane >> 2 & 7u)) * 0x10001u;
   uint32_t lmax = 0;
   int zeros = 0, starts = 0;
   uint32_t zmask[8];  // zero bits of each chunk's eight symbols
@@ -595,10 +600,9 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
   uint32_t prev_last = 0;  // symbol before lane 0's chunk is zero
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    ch[c].hist_add(row0, row1);
+    uint32_t zm = ch[c].hist_add(row0, row1, junk);
     lmax = max(lmax, ch[c].max8());
     orv[c] = ch[c].or8();
-    uint32_t zm = ch[c].zero_bits();
     if (!full) zm &= (1u << clampi(count - (512 * c + 8 * lane), 0, 8)) - 1u;
     zmask[c] = zm;
     zeros += __popc(zm);
@@ -622,10 +626,21 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
   uint8_t* sbase = scratch + (int64_t)s * kScratchBytes;
 
   // ---------------- present values by rank
-  // bins 8 lane + k, copies summed; the padding zeros leave class r's zero
-  // bin (1024 - (count - r + 3) / 4 of them)
+  // bins 8 lane + k, copies summed; bin 0 from the zero masks (padding
+  // excluded), class r's zeros at bits r and r + 4 of each chunk's mask
   int distinct = 0;
   if (exact) {
+    const uint32_t z03 = zmask[0] | (zmask[1] << 8) | (zmask[2] << 16) | (zmask[3] << 24);
+    const uint32_t z47 = zmask[4] | (zmask[5] << 8) | (zmask[6] << 16) | (zmask[7] << 24);
+    uint32_t zc[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t m = 0x11111111u << r;
+      zc[r] = (uint32_t)(__popc(z03 & m) + __popc(z47 & m));
+    }
+    // class pairs as 16-bit halves (at most 1024 zeros a class)
+    const uint32_t zero01 = wave_sum(zc[0] | (zc[1] << 16));
+    const uint32_t zero23 = wave_sum(zc[2] | (zc[3] << 16));
     uint32_t c01[8], c23[8];
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
@@ -639,13 +654,9 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
         for (int k = 0; k < 8; ++k) acc[k] = cp == 0 ? w[k] : acc[k] + w[k];
       }
     }
-    if (!full && lane == 0) {
-      const uint32_t pad0 = 1024u - (uint32_t)((count + 3) >> 2);
-      const uint32_t pad1 = 1024u - (uint32_t)((count + 2) >> 2);
-      const uint32_t pad2 = 1024u - (uint32_t)((count + 1) >> 2);
-      const uint32_t pad3 = 1024u - (uint32_t)(count >> 2);
-      c01[0] -= pad0 | (pad1 << 16);
-      c23[0] -= pad2 | (pad3 << 16);
+    if (lane == 0) {
+      c01[0] = zero01;
+      c23[0] = zero23;
     }
     uint32_t cnt[8];
     uint32_t pm = 0;
@@ -1110,7 +1121,10 @@ __global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frame
                                              const uint8_t* __restrict__ rank_area,
                                              int cap_below) {
   (void)folded;
-  __shared__ uint4 ent_all[kRansShards][kCap];
+  // one entry of padding per row: the shards' rows start four banks apart,
+  // so the sixteen shards' lookups of their commonest ranks do not all meet
+  // on the same banks
+  __shared__ uint4 ent_all[kRansShards][kCap + 1];
   const int groups = (shard_total + kRansShards - 1) / kRansShards;
   for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
     rans_group<FT>(frames, states, tiles, shards, shard_total, recs, scratch, rank_area,
