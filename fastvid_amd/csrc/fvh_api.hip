@@ -115,7 +115,7 @@ Geometry fill_geometry(FrameDesc& f, int64_t width, int64_t height, int64_t tw, 
 
 struct WorkspaceLayout {
   int64_t frames, states, tiles, shards, recs, tile_len, tile_off, folded, scratch, ranks, recon,
-      dshards, sink, queues, groups;
+      dshards, sink, queues, groups, emit_list;
   int64_t total;
 };
 
@@ -313,6 +313,7 @@ int build_encode_plan(const fvh_encode_frame* in, int32_t count, EncodePlan& pla
   w.sink = take(kSinkBytes);
   w.queues = take(kQueueCount * 4);
   w.groups = take(plan.groups * 4);
+  w.emit_list = take(plan.shards * 4);
   w.total = off;
   return FVH_OK;
 }
@@ -367,6 +368,7 @@ int fvh_encode_launch(const fvh_encode_frame* frames, int32_t frame_count, void*
   b.sink = ws + plan.ws.sink;
   b.queues = (int32_t*)(ws + plan.ws.queues);
   b.group_first = (int32_t*)(ws + plan.ws.groups);
+  b.emit_list = (int32_t*)(ws + plan.ws.emit_list);
   b.group_total = (int)plan.groups;
   b.band_ring = plan.band_ring;
   b.results = (Results*)results;

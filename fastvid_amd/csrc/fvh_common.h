@@ -31,8 +31,12 @@ constexpr uint32_t kRansL = 1u << 23;
 // plus the 3-byte record header.
 constexpr int kMaxRecordBytes = 8739;
 
-// Per-shard order-0 scratch layout (bytes).
-constexpr int kScratchTable = 0;        // 256 x {freq | cum << 13, 0} by rank
+// Per-shard order-0 scratch layout (bytes).  Once the rANS encoder has read
+// the entries, it writes the order-0 record's head around the sparse table:
+// [mode][body u16] at kScratchHead, and the lane lengths 0..2 and final
+// states (7 x u32) over the first entries, at kScratchTable.
+constexpr int kScratchTable = 0;        // 255 x {freq | cum << 13, 0} by rank
+constexpr int kScratchHead = 2045;      // 3 record header bytes, just before the sparse table
 constexpr int kScratchSparse = 2048;    // sparse table bytes (<= 1024)
 constexpr int kScratchLanes = 3072;     // 4 lanes x kLaneBytes, filled from the end
 constexpr int kLaneBytes = 1536;        // proven bound for 1024 symbols
@@ -141,6 +145,7 @@ struct TileDesc {
 struct ShardDesc {
   int32_t tile;            // global tile index
   int32_t local;           // shard index within the tile
+  int32_t frame;           // batch frame index (the tile's)
 };
 
 // Encoder per-shard analysis record.

@@ -12,7 +12,8 @@
 //   k_tile_scan    record offsets inside each tile, tile payload lengths
 //   k_frame_finish tile offsets, frame payload, header + directory
 //   k_decide       gray10 MED / gray16 refinement decisions between passes
-//   k_emit         record assembly in LDS, coalesced copy to the stream
+//   k_emit         zero-run / Rice / block-pack records assembled in LDS, copied
+//                  to the stream; k_emit_o0 copies order-0 records from scratch
 #include "fvh_device.h"
 #include "fvh_kernels.h"
 #include "fvh_timing.h"
@@ -1096,16 +1097,28 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   const int n1 = shfl(nbytes, base + 1);
   const int n2 = shfl(nbytes, base + 2);
   const int n3 = shfl(nbytes, base + 3);
-  if (act && r == 0) {
-    ShardRec& rec = recs[s];
-    const uint32_t body0 = (uint32_t)rec.sparse_len + 28u + (uint32_t)(n0 + n1 + n2 + n3);
-    if (frames[td.frame].force_order0 || body0 < rec.best3_body) {
-      rec.mode = kModeOrder0;
-      rec.body = body0;
-      rec.lane[0] = (uint16_t)n0;
-      rec.lane[1] = (uint16_t)n1;
-      rec.lane[2] = (uint16_t)n2;
-      rec.lane[3] = (uint16_t)n3;
+  if (act) {
+    // the record's head for k_emit_o0: lane lengths 0..2 and final states
+    // over the first (already read) entries, [mode][body u16] before the
+    // sparse table
+    const uint32_t body0 =
+        (uint32_t)recs[s].sparse_len + 28u + (uint32_t)(n0 + n1 + n2 + n3);
+    uint32_t* const head = (uint32_t*)(sbase + kScratchTable);
+    if (r < 3) head[r] = (uint32_t)nbytes;
+    head[3 + r] = x;
+    if (r == 0) {
+      sbase[kScratchHead] = (uint8_t)kModeOrder0;
+      sbase[kScratchHead + 1] = (uint8_t)body0;
+      sbase[kScratchHead + 2] = (uint8_t)(body0 >> 8);
+      ShardRec& rec = recs[s];
+      if (frames[td.frame].force_order0 || body0 < rec.best3_body) {
+        rec.mode = kModeOrder0;
+        rec.body = body0;
+        rec.lane[0] = (uint16_t)n0;
+        rec.lane[1] = (uint16_t)n1;
+        rec.lane[2] = (uint16_t)n2;
+        rec.lane[3] = (uint16_t)n3;
+      }
     }
   }
 }
@@ -1135,10 +1148,15 @@ __global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frame
 // ---------------------------------------------------------------- assembly
 
 // Record offsets inside each tile (records are [mode u8][body u16][body]).
+// The final pass (all) also lists the shards whose records are not order-0,
+// for k_emit (one atomic per wave and 64 shards).  A record is at most
+// 3 + 65535 bytes, so 64 of them sum in 32 bits.
 __global__ __launch_bounds__(256) void k_tile_scan(FrameState* __restrict__ states,
                                                    const TileDesc* __restrict__ tiles,
                                                    int tile_total, ShardRec* __restrict__ recs,
-                                                   int64_t* __restrict__ tile_len, int all) {
+                                                   int64_t* __restrict__ tile_len, int all,
+                                                   int32_t* __restrict__ emit_list,
+                                                   int32_t* __restrict__ emit_count) {
   const int lane = lane_id();
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= tile_total) return;
@@ -1147,10 +1165,23 @@ __global__ __launch_bounds__(256) void k_tile_scan(FrameState* __restrict__ stat
   int64_t carry = 0;
   for (int c0 = 0; c0 < td.shard_count; c0 += 64) {
     const int i = c0 + lane;
-    const int64_t rb = i < td.shard_count ? (int64_t)recs[td.first_shard + i].body + 3 : 0;
-    const int64_t incl = wave_incl_scan(rb);
-    if (i < td.shard_count) recs[td.first_shard + i].offset = (uint32_t)(carry + incl - rb);
-    carry += shfl(incl, 63);
+    const bool in = i < td.shard_count;
+    const ShardRec* rec = &recs[td.first_shard + (in ? i : 0)];
+    const uint32_t rb = in ? rec->body + 3u : 0u;
+    const uint32_t incl = wave_incl_scan(rb);
+    if (in) recs[td.first_shard + i].offset = (uint32_t)(carry + incl - rb);
+    carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    if (all) {
+      const bool other = in && rec->mode != kModeOrder0;
+      const uint64_t m = ballot(other);
+      if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(emit_count, popc64(m));
+        base = shfl(base, leader);
+        if (other) emit_list[base + popc64(m & ((1ull << lane) - 1))] = td.first_shard + i;
+      }
+    }
   }
   if (lane == 0) tile_len[t] = carry;
 }
@@ -1259,32 +1290,31 @@ __device__ __forceinline__ void or_bits(uint32_t* buf, uint32_t pos, uint32_t pa
   if ((uint32_t)(pp >> 32)) atomicOr(&buf[w + 1], (uint32_t)(pp >> 32));
 }
 
-// One wave64 per shard builds the record [mode][body u16][body] in LDS and
-// copies it to its offset; the reference instead sets bits with global
-// atomics into a zero-filled stream (encode.cu:458-474, 906).  The record
-// sits in LDS at its destination's offset within a 16-byte chunk, so the
-// copy is one 16-byte store per chunk it fills, and byte stores only in the
-// first and last chunks, which it shares with the neighbouring records.
+// Zero-run, Rice and block-pack records: one wave64 per shard builds the
+// record [mode][body u16][body] in LDS and copies it to its offset; the
+// reference instead sets bits with global atomics into a zero-filled stream
+// (encode.cu:458-474, 906).  The record sits in LDS at its destination's
+// offset within a 16-byte chunk, so the copy is one 16-byte store per chunk
+// it fills, and byte stores only in the first and last chunks, which it
+// shares with the neighbouring records.  Order-0 records are k_emit_o0's;
+// k_emit's waves walk the list of the other shards (final k_tile_scan), so
+// a batch without such records costs one read of the list's length.
 template <typename FT>
-__global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ frames,
-                                              const FrameState* __restrict__ states,
-                                              const TileDesc* __restrict__ tiles,
-                                              const ShardDesc* __restrict__ shards,
-                                              int shard_total, const FT* __restrict__ folded,
-                                              const ShardRec* __restrict__ recs,
-                                              const uint8_t* __restrict__ scratch,
-                                              const int64_t* __restrict__ tile_off, char* gbase) {
-  __shared__ __align__(16) uint32_t buf_all[4][kBufWords];
-  __shared__ uint64_t zm_all[4][65];
+__device__ __forceinline__ void emit_record(int s, const FrameDesc* __restrict__ frames,
+                                            const FrameState* __restrict__ states,
+                                            const TileDesc* __restrict__ tiles,
+                                            const ShardDesc* __restrict__ shards,
+                                            const FT* __restrict__ folded,
+                                            const ShardRec* __restrict__ recs,
+                                            const int64_t* __restrict__ tile_off, char* gbase,
+                                            uint32_t* buf, uint64_t* zm) {
   const int lane = lane_id();
-  const int wave = threadIdx.x >> 6;
-  const int s = blockIdx.x * 4 + wave;
-  if (s >= shard_total) return;
+  const ShardRec rec = recs[s];
+  if (rec.mode == kModeOrder0) return;
   const ShardDesc sd = shards[s];
   const TileDesc td = tiles[sd.tile];
   if (states[td.frame].status != 0) return;
   const FrameDesc& f = frames[td.frame];
-  const ShardRec rec = recs[s];
   const int64_t local0 = (int64_t)sd.local * kShardSymbols;
   const int count = (int)min((int64_t)kShardSymbols, (int64_t)td.width * td.height - local0);
   const FT* v = folded + td.folded_base + local0;
@@ -1292,7 +1322,6 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
   const int total = 3 + (int)rec.body;
   const int sh = (int)((uintptr_t)out & 15);
   const uint32_t hdr = 3u + (uint32_t)sh;  // body's first byte in buf
-  uint32_t* buf = buf_all[wave];
   uint8_t* bytes = (uint8_t*)buf + sh;
   uint8_t* body = bytes + 3;
   const int words = (sh + total + 3) >> 2;
@@ -1303,7 +1332,6 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
   if (mode == kModeZeroRun) {
     // zero-run tokens (encode.cu:514-536): a nonzero v is varint(2v-1), a
     // maximal zero run of R symbols is varint(2(R-1)) at the run's start.
-    uint64_t* zm = zm_all[wave];
     for (int j = 0; j < 64; ++j) {
       const int i = (j << 6) + lane;
       const uint64_t Z = ballot(i < count && v[i] == 0);
@@ -1424,49 +1452,6 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
         or_bits(buf, pos, (uint32_t)v[i]);
       }
     }
-  } else {
-    // order-0 (encode.cu:587-611): sparse table, lane lengths 0..2, final
-    // states, lane bytes in decode order.
-    const uint8_t* sbase = scratch + (int64_t)s * kScratchBytes;
-    const int sp = rec.sparse_len;
-    // body bytes [0, sp) are the sparse table and [d1, d5) the four lanes'
-    // bytes, lane r's ending at the end of its scratch area: one gather with
-    // sixteen independent loads in flight per thread (the 28 bytes between
-    // come from the record)
-    const int d1 = sp + 28;
-    const int d2 = d1 + rec.lane[0];
-    const int d3 = d2 + rec.lane[1];
-    const int d4 = d3 + rec.lane[2];
-    const int d5 = d4 + rec.lane[3];
-    const int src_off[5] = {kScratchSparse, kScratchLanes + 1 * kLaneBytes - d2,
-                            kScratchLanes + 2 * kLaneBytes - d3,
-                            kScratchLanes + 3 * kLaneBytes - d4,
-                            kScratchLanes + 4 * kLaneBytes - d5};
-    for (int b0 = 0; b0 < d5; b0 += 1024) {
-      uint32_t val[16];
-#pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        const int i = b0 + 64 * u + lane;
-        const int piece = (i >= d1) + (i >= d2) + (i >= d3) + (i >= d4);
-        const int so = piece == 0 ? src_off[0] : piece == 1 ? src_off[1] : piece == 2 ? src_off[2]
-                       : piece == 3 ? src_off[3] : src_off[4];
-        const bool in = i < sp || (i >= d1 && i < d5);
-        val[u] = in ? sbase[so + i] : 0u;
-      }
-#pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        const int i = b0 + 64 * u + lane;
-        if (i < sp || (i >= d1 && i < d5)) body[i] = (uint8_t)val[u];
-      }
-    }
-    if (lane < 3) {
-      const uint32_t len = rec.lane[lane];
-      for (int q = 0; q < 4; ++q) body[sp + lane * 4 + q] = (uint8_t)(len >> (8 * q));
-    }
-    if (lane < 4) {
-      const uint32_t stt = rec.states[lane];
-      for (int q = 0; q < 4; ++q) body[sp + 12 + lane * 4 + q] = (uint8_t)(stt >> (8 * q));
-    }
   }
   __builtin_amdgcn_wave_barrier();
   if (lane == 0) {
@@ -1483,6 +1468,98 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
     } else {
       for (int j = max(sh - 16 * c, 0); j < 16 && 16 * c + j < end; ++j) {
         out16[16 * c + j] = ((const uint8_t*)buf)[16 * c + j];
+      }
+    }
+  }
+}
+
+template <typename FT>
+__global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ frames,
+                                              const FrameState* __restrict__ states,
+                                              const TileDesc* __restrict__ tiles,
+                                              const ShardDesc* __restrict__ shards,
+                                              const int32_t* __restrict__ emit_list,
+                                              const int32_t* __restrict__ emit_count,
+                                              const FT* __restrict__ folded,
+                                              const ShardRec* __restrict__ recs,
+                                              const int64_t* __restrict__ tile_off, char* gbase) {
+  __shared__ __align__(16) uint32_t buf_all[4][kBufWords];
+  __shared__ uint64_t zm_all[4][65];
+  const int wave = threadIdx.x >> 6;
+  const int n = *emit_count;
+  for (int i = blockIdx.x * 4 + wave; i < n; i += 4 * (int)gridDim.x) {
+    // the wave's LDS image is rebuilt for each record: keep the previous
+    // record's reads ahead of the next one's writes
+    asm volatile("" ::: "memory");
+    emit_record<FT>(emit_list[i], frames, states, tiles, shards, folded, recs, tile_off, gbase,
+                    buf_all[wave], zm_all[wave]);
+  }
+}
+
+// Order-0 records (encode.cu:587-611), straight from the shard's scratch
+// with no LDS staging: the record is [mode][body u16][sparse table] at
+// kScratchHead, then the lane lengths 0..2 and final states at
+// kScratchTable (both written by k_rans), then the four lanes' bytes at the
+// ends of their areas.  Lane l writes destination 16-byte chunks l, l + 64,
+// ...: a whole chunk inside one of these pieces is five dwords loaded and
+// byte-aligned, and the chunks that cross a piece boundary or the record's
+// ends (shared with the neighbouring records) go byte by byte.
+template <typename FT>
+__global__ __launch_bounds__(256) void k_emit_o0(const FrameDesc* __restrict__ frames,
+                                                 const FrameState* __restrict__ states,
+                                                 const ShardDesc* __restrict__ shards,
+                                                 int shard_total, const ShardRec* __restrict__ recs,
+                                                 const uint8_t* __restrict__ scratch,
+                                                 const int64_t* __restrict__ tile_off, char* gbase) {
+  using u4a = uint4 __attribute__((aligned(4)));
+  const int lane = lane_id();
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= shard_total) return;
+  const ShardRec rec = recs[s];
+  const ShardDesc sd = shards[s];
+  if (rec.mode != kModeOrder0) return;
+  if (states[sd.frame].status != 0) return;
+  uint8_t* const out = as_global(gbase, frames[sd.frame].stream + tile_off[sd.tile] + rec.offset);
+  const uint8_t* const sb = scratch + (int64_t)s * kScratchBytes;
+  // piece ends in record bytes, and each piece's scratch offset less its
+  // first record byte
+  const int e0 = 3 + rec.sparse_len;
+  const int e1 = e0 + 28;
+  const int e2 = e1 + rec.lane[0];
+  const int e3 = e2 + rec.lane[1];
+  const int e4 = e3 + rec.lane[2];
+  const int total = e4 + rec.lane[3];  // 3 + rec.body
+  const int so[6] = {kScratchHead, kScratchTable - e0, kScratchLanes + 1 * kLaneBytes - e2,
+                     kScratchLanes + 2 * kLaneBytes - e3, kScratchLanes + 3 * kLaneBytes - e4,
+                     kScratchLanes + 4 * kLaneBytes - total};
+  auto piece = [&](int o) __attribute__((always_inline)) {
+    return (o >= e0) + (o >= e1) + (o >= e2) + (o >= e3) + (o >= e4);
+  };
+  auto src_off = [&](int p) __attribute__((always_inline)) {
+    return p == 0 ? so[0] : p == 1 ? so[1] : p == 2 ? so[2] : p == 3 ? so[3] : p == 4 ? so[4] : so[5];
+  };
+  const int sh = (int)((uintptr_t)out & 15);
+  uint8_t* const out16 = out - sh;
+  const int end = sh + total;
+  for (int c = lane; 16 * c < end; c += 64) {
+    const int o0 = 16 * c - sh;  // record byte of the chunk's first byte
+    const int p = piece(o0);
+    if (o0 >= 0 && o0 + 16 <= total && piece(o0 + 15) == p) {
+      const uint8_t* src = sb + src_off(p) + o0;
+      const uint32_t k = (uint32_t)((uintptr_t)src & 3);
+      const uint32_t* a = (const uint32_t*)(src - k);
+      const uint4 w = *(const u4a*)a;
+      // the fifth dword holds byte 15 when the source is not dword aligned
+      const uint32_t w4 = k ? a[4] : 0u;
+      uint4 v;
+      v.x = __builtin_amdgcn_alignbyte(w.y, w.x, k);
+      v.y = __builtin_amdgcn_alignbyte(w.z, w.y, k);
+      v.z = __builtin_amdgcn_alignbyte(w.w, w.z, k);
+      v.w = __builtin_amdgcn_alignbyte(w4, w.w, k);
+      ((uint4*)out16)[c] = v;
+    } else {
+      for (int j = max(-o0, 0); j < 16 && o0 + j < total; ++j) {
+        out16[16 * c + j] = sb[src_off(piece(o0 + j)) + o0 + j];
       }
     }
   }
@@ -1552,7 +1629,7 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
     {
       FVH_TIMED("k_tile_scan", stream);
       hipLaunchKernelGGL(k_tile_scan, tgrid, dim3(256), 0, stream, b.states, b.tiles, b.tile_total,
-                         b.recs, b.tile_len, 0);
+                         b.recs, b.tile_len, 0, (int32_t*)nullptr, (int32_t*)nullptr);
     }
     {
       FVH_TIMED("k_frame_finish", stream);
@@ -1568,7 +1645,8 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
       {
         FVH_TIMED("k_tile_scan", stream);
         hipLaunchKernelGGL(k_tile_scan, tgrid, dim3(256), 0, stream, b.states, b.tiles,
-                           b.tile_total, b.recs, b.tile_len, 0);
+                           b.tile_total, b.recs, b.tile_len, 0, (int32_t*)nullptr,
+                           (int32_t*)nullptr);
       }
       {
         FVH_TIMED("k_frame_finish", stream);
@@ -1590,7 +1668,7 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
   {
     FVH_TIMED("k_tile_scan", stream);
     hipLaunchKernelGGL(k_tile_scan, tgrid, dim3(256), 0, stream, b.states, b.tiles, b.tile_total,
-                       b.recs, b.tile_len, 1);
+                       b.recs, b.tile_len, 1, b.emit_list, b.queues + kEmitQueue);
   }
   {
     FVH_TIMED("k_frame_finish", stream);
@@ -1599,9 +1677,16 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
   }
   {
     FVH_TIMED("k_emit", stream);
-    hipLaunchKernelGGL((k_emit<FT>), dim3((b.shard_total + 3) / 4), dim3(256), 0, stream, b.frames,
-                       b.states, b.tiles, b.shards, b.shard_total, (const FT*)b.folded, b.recs,
-                       b.scratch, b.tile_off, (char*)b.folded);
+    hipLaunchKernelGGL((k_emit<FT>), dim3(min((b.shard_total + 3) / 4, 4 * device_cus())),
+                       dim3(256), 0, stream, b.frames, b.states, b.tiles, b.shards, b.emit_list,
+                       b.queues + kEmitQueue, (const FT*)b.folded, b.recs, b.tile_off,
+                       (char*)b.folded);
+  }
+  {
+    FVH_TIMED("k_emit_o0", stream);
+    hipLaunchKernelGGL((k_emit_o0<FT>), dim3((b.shard_total + 3) / 4), dim3(256), 0, stream,
+                       b.frames, b.states, b.shards, b.shard_total, b.recs, b.scratch, b.tile_off,
+                       (char*)b.folded);
   }
 }
 

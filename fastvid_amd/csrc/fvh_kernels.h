@@ -10,6 +10,7 @@ constexpr int kBandRingMax = 512;  // widest tile handled by the band wavefront
 constexpr int kLaneSinkBytes = 32; // per-lane sink slot for masked-off stores (8 x uint32)
 constexpr int kSinkBytes = 256 * kLaneSinkBytes;  // workspace sink region
 constexpr int kQueueCount = 8;     // work counters per call (two per predictor pass)
+constexpr int kEmitQueue = 7;      // encoder: length of the emit list
 
 struct EncodeBuffers {
   FrameDesc* frames;
@@ -26,6 +27,8 @@ struct EncodeBuffers {
   uint8_t* sink;         // kSinkBytes target of masked-off stores
   int32_t* queues;       // kQueueCount work counters, zeroed by k_state_init
   int32_t* group_first;  // first tile of each band tile group (written by k_plan)
+  int32_t* emit_list;    // shards whose records are not order-0 (final k_tile_scan),
+                         // counted in queues[kEmitQueue]
   int group_total;
   int band_ring;         // ring slots per tile of the band wavefront (widest band tile)
   Results* results;

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(256) void k_plan(const FrameDesc* __restrict__ fram
     ShardDesc sd;
     sd.tile = t;
     sd.local = (int32_t)s;
+    sd.frame = lo;
     shards[first + s] = sd;
   }
 }
