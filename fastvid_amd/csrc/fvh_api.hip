@@ -306,7 +306,8 @@ int build_encode_plan(const fvh_encode_frame* in, int32_t count, EncodePlan& pla
   w.tile_len = take(plan.tiles * 8);
   w.tile_off = take(plan.tiles * 8);
   w.folded = take((plan.samples + 8) * (plan.wide_folded ? 4 : 2));
-  w.scratch = take(plan.shards * (int64_t)kScratchBytes);
+  // (64 bytes of slack: order-0 record copies read a few bytes past a piece)
+  w.scratch = take(plan.shards * (int64_t)kScratchBytes + 64);
   w.ranks = take((plan.shards + kRansShards - 1) / kRansShards * (int64_t)kRankGroupBytes);
   w.recon = plan.any_wide ? take(plan.samples * 2) : -1;
   w.dshards = -1;

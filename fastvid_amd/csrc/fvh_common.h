@@ -34,8 +34,11 @@ constexpr int kMaxRecordBytes = 8739;
 // Per-shard order-0 scratch layout (bytes).  Once the rANS encoder has read
 // the entries, it writes the order-0 record's head around the sparse table:
 // [mode][body u16] at kScratchHead, and the lane lengths 0..2 and final
-// states (7 x u32) over the first entries, at kScratchTable.
+// states (7 x u32) over spent entries at kScratchLensStates (not at 0, so a
+// dword-aligned 16-byte read that starts up to 15 bytes before them stays in
+// the shard).
 constexpr int kScratchTable = 0;        // 255 x {freq | cum << 13, 0} by rank
+constexpr int kScratchLensStates = 20;  // 28 bytes of the order-0 record
 constexpr int kScratchHead = 2045;      // 3 record header bytes, just before the sparse table
 constexpr int kScratchSparse = 2048;    // sparse table bytes (<= 1024)
 constexpr int kScratchLanes = 3072;     // 4 lanes x kLaneBytes, filled from the end

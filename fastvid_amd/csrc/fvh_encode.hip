@@ -1099,11 +1099,10 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   const int n3 = shfl(nbytes, base + 3);
   if (act) {
     // the record's head for k_emit_o0: lane lengths 0..2 and final states
-    // over the first (already read) entries, [mode][body u16] before the
-    // sparse table
+    // over spent entries, [mode][body u16] before the sparse table
     const uint32_t body0 =
         (uint32_t)recs[s].sparse_len + 28u + (uint32_t)(n0 + n1 + n2 + n3);
-    uint32_t* const head = (uint32_t*)(sbase + kScratchTable);
+    uint32_t* const head = (uint32_t*)(sbase + kScratchLensStates);
     if (r < 3) head[r] = (uint32_t)nbytes;
     head[3 + r] = x;
     if (r == 0) {
@@ -1499,11 +1498,24 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
 // Order-0 records (encode.cu:587-611), straight from the shard's scratch
 // with no LDS staging: the record is [mode][body u16][sparse table] at
 // kScratchHead, then the lane lengths 0..2 and final states at
-// kScratchTable (both written by k_rans), then the four lanes' bytes at the
-// ends of their areas.  Lane l writes destination 16-byte chunks l, l + 64,
-// ...: a whole chunk inside one of these pieces is five dwords loaded and
-// byte-aligned, and the chunks that cross a piece boundary or the record's
-// ends (shared with the neighbouring records) go byte by byte.
+// kScratchLensStates (both written by k_rans), then the four lanes' bytes
+// at the ends of their areas.  Sixteen lanes per record, four records per
+// wave (a wave's descriptor loads serve four records); lane j of a record
+// writes destination 16-byte chunks j, j + 16, ....  A chunk takes, from
+// each piece it overlaps (usually one), sixteen bytes loaded as five dwords
+// and byte-aligned, merged under a byte mask; only the chunks at the
+// record's two ends, shared with the neighbouring records, store bytes.
+// The loads reach at most 15 bytes (plus a dword's remainder) outside a
+// piece, which the scratch layout keeps inside the allocation.
+__device__ __forceinline__ uint32_t byte_span_mask(int lo, int hi, int d) {
+  // bytes [lo, hi) of a 16-byte chunk, restricted to its dword d
+  const int l = clampi(lo - 4 * d, 0, 4);
+  const int h = clampi(hi - 4 * d, 0, 4);
+  const uint32_t below_h = h >= 4 ? ~0u : (1u << (8 * h)) - 1u;
+  const uint32_t below_l = l >= 4 ? ~0u : (1u << (8 * l)) - 1u;
+  return below_h & ~below_l;
+}
+
 template <typename FT>
 __global__ __launch_bounds__(256) void k_emit_o0(const FrameDesc* __restrict__ frames,
                                                  const FrameState* __restrict__ states,
@@ -1512,8 +1524,8 @@ __global__ __launch_bounds__(256) void k_emit_o0(const FrameDesc* __restrict__ f
                                                  const uint8_t* __restrict__ scratch,
                                                  const int64_t* __restrict__ tile_off, char* gbase) {
   using u4a = uint4 __attribute__((aligned(4)));
-  const int lane = lane_id();
-  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int j = (int)threadIdx.x & 15;
+  const int s = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
   if (s >= shard_total) return;
   const ShardRec rec = recs[s];
   const ShardDesc sd = shards[s];
@@ -1521,45 +1533,59 @@ __global__ __launch_bounds__(256) void k_emit_o0(const FrameDesc* __restrict__ f
   if (states[sd.frame].status != 0) return;
   uint8_t* const out = as_global(gbase, frames[sd.frame].stream + tile_off[sd.tile] + rec.offset);
   const uint8_t* const sb = scratch + (int64_t)s * kScratchBytes;
-  // piece ends in record bytes, and each piece's scratch offset less its
-  // first record byte
+  // piece p covers record bytes [start(p), end(p)); its byte o is at
+  // scratch offset off(p) + o
   const int e0 = 3 + rec.sparse_len;
   const int e1 = e0 + 28;
   const int e2 = e1 + rec.lane[0];
   const int e3 = e2 + rec.lane[1];
   const int e4 = e3 + rec.lane[2];
   const int total = e4 + rec.lane[3];  // 3 + rec.body
-  const int so[6] = {kScratchHead, kScratchTable - e0, kScratchLanes + 1 * kLaneBytes - e2,
-                     kScratchLanes + 2 * kLaneBytes - e3, kScratchLanes + 3 * kLaneBytes - e4,
-                     kScratchLanes + 4 * kLaneBytes - total};
+  auto sel = [](int p, int a0, int a1, int a2, int a3, int a4, int a5) __attribute__((always_inline)) {
+    return p == 0 ? a0 : p == 1 ? a1 : p == 2 ? a2 : p == 3 ? a3 : p == 4 ? a4 : a5;
+  };
   auto piece = [&](int o) __attribute__((always_inline)) {
     return (o >= e0) + (o >= e1) + (o >= e2) + (o >= e3) + (o >= e4);
-  };
-  auto src_off = [&](int p) __attribute__((always_inline)) {
-    return p == 0 ? so[0] : p == 1 ? so[1] : p == 2 ? so[2] : p == 3 ? so[3] : p == 4 ? so[4] : so[5];
   };
   const int sh = (int)((uintptr_t)out & 15);
   uint8_t* const out16 = out - sh;
   const int end = sh + total;
-  for (int c = lane; 16 * c < end; c += 64) {
+  for (int c = j; 16 * c < end; c += 16) {
     const int o0 = 16 * c - sh;  // record byte of the chunk's first byte
-    const int p = piece(o0);
-    if (o0 >= 0 && o0 + 16 <= total && piece(o0 + 15) == p) {
-      const uint8_t* src = sb + src_off(p) + o0;
+    const int tlo = max(-o0, 0);
+    const int thi = min(total - o0, 16);
+    const int p_last = piece(o0 + thi - 1);
+    uint4 v = make_uint4(0, 0, 0, 0);
+    for (int p = piece(o0 + tlo); p <= p_last; ++p) {
+      const int lo = max(tlo, sel(p, 0, e0, e1, e2, e3, e4) - o0);
+      const int hi = min(thi, sel(p, e0, e1, e2, e3, e4, total) - o0);
+      if (lo >= hi) continue;  // an empty piece (a lane without bytes)
+      const int off = sel(p, kScratchHead, kScratchLensStates - e0,
+                          kScratchLanes + 1 * kLaneBytes - e2, kScratchLanes + 2 * kLaneBytes - e3,
+                          kScratchLanes + 3 * kLaneBytes - e4, kScratchLanes + 4 * kLaneBytes - total);
+      const uint8_t* src = sb + off + o0;
       const uint32_t k = (uint32_t)((uintptr_t)src & 3);
-      const uint32_t* a = (const uint32_t*)(src - k);
-      const uint4 w = *(const u4a*)a;
-      // the fifth dword holds byte 15 when the source is not dword aligned
-      const uint32_t w4 = k ? a[4] : 0u;
-      uint4 v;
-      v.x = __builtin_amdgcn_alignbyte(w.y, w.x, k);
-      v.y = __builtin_amdgcn_alignbyte(w.z, w.y, k);
-      v.z = __builtin_amdgcn_alignbyte(w.w, w.z, k);
-      v.w = __builtin_amdgcn_alignbyte(w4, w.w, k);
+      const uint32_t* w = (const uint32_t*)(src - k);
+      const uint4 q = *(const u4a*)w;
+      const uint32_t q4 = w[4];
+      const uint32_t x0 = __builtin_amdgcn_alignbyte(q.y, q.x, k);
+      const uint32_t x1 = __builtin_amdgcn_alignbyte(q.z, q.y, k);
+      const uint32_t x2 = __builtin_amdgcn_alignbyte(q.w, q.z, k);
+      const uint32_t x3 = __builtin_amdgcn_alignbyte(q4, q.w, k);
+      const uint32_t m0 = byte_span_mask(lo, hi, 0), m1 = byte_span_mask(lo, hi, 1);
+      const uint32_t m2 = byte_span_mask(lo, hi, 2), m3 = byte_span_mask(lo, hi, 3);
+      v.x = (x0 & m0) | (v.x & ~m0);
+      v.y = (x1 & m1) | (v.y & ~m1);
+      v.z = (x2 & m2) | (v.z & ~m2);
+      v.w = (x3 & m3) | (v.w & ~m3);
+    }
+    if (tlo == 0 && thi == 16) {
       ((uint4*)out16)[c] = v;
     } else {
-      for (int j = max(-o0, 0); j < 16 && o0 + j < total; ++j) {
-        out16[16 * c + j] = sb[src_off(piece(o0 + j)) + o0 + j];
+      const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        if (t >= tlo && t < thi) out16[16 * c + t] = (uint8_t)(vw[t >> 2] >> (8 * (t & 3)));
       }
     }
   }
@@ -1684,7 +1710,7 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
   }
   {
     FVH_TIMED("k_emit_o0", stream);
-    hipLaunchKernelGGL((k_emit_o0<FT>), dim3((b.shard_total + 3) / 4), dim3(256), 0, stream,
+    hipLaunchKernelGGL((k_emit_o0<FT>), dim3((b.shard_total + 15) / 16), dim3(256), 0, stream,
                        b.frames, b.states, b.shards, b.shard_total, b.recs, b.scratch, b.tile_off,
                        (char*)b.folded);
   }
