@@ -38,6 +38,7 @@ PROFILE_NAMES = {
     "k_analyze": "k_analyze<u16>",
     "k_rans": "k_rans<u16,",
     "k_emit": "k_emit<u16>",
+    "k_emit_o0": "k_emit_o0<u16>",
     "k_dec_o0": "k_dec_o0<u16,",
     "k_dec_other": "k_dec_other<u16>",
     "k_o0_parse": "k_o0_parse",
@@ -106,7 +107,8 @@ def algorithmic_bytes(samples, stream_bytes, split):
         "k_wavefront_enc": 2 * s + 2 * s,
         "k_analyze": 2 * s,                 # residuals in
         "k_rans": 2 * s + b,                # residuals in, lane bytes out
-        "k_emit": 2 * s + 2 * b,            # residuals + lane bytes in, stream out
+        "k_emit": 2 * ot_s + 2 * ot_b,      # other records: residuals in, records out
+        "k_emit_o0": 2 * o0_b,              # order-0 records: scratch pieces in, records out
         "k_dec_o0": o0_b + 2 * o0_s,        # order-0 bodies in, residuals out
         "k_dec_other": ot_b + 2 * ot_s,     # other bodies in, residuals out
         "k_band_dec": 2 * s + 2 * s,        # residuals in, samples out
