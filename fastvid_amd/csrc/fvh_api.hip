@@ -554,8 +554,9 @@ static DecodePlanLayout decode_layout(const fvh_stream_info* infos, int32_t coun
   L.tiles = take(L.tiles_n * (int64_t)sizeof(TileDesc));
   L.shards = take(L.shards_n * (int64_t)sizeof(ShardDesc));
   L.dshards = take(L.shards_n * (int64_t)sizeof(DShard));
-  // order-0 list, other list, three order-0 tier lists, then the counters
-  L.lists = take((5 * L.shards_n + 8) * 4);
+  // order-0 list, other list, three order-0 tier lists, Rice list, then the
+  // counters
+  L.lists = take((6 * L.shards_n + 8) * 4);
   L.o0tabs = take(L.shards_n * (int64_t)sizeof(O0Tab));
   L.folded = take((L.samples + 8) * (L.wide_folded ? 4 : 2));
   L.sink = take(kSinkBytes);
@@ -656,7 +657,7 @@ int decode_launch(const fvh_stream_info* infos, const void* const* streams,
   b.shards = (ShardDesc*)(ws + L.shards);
   b.dshards = (DShard*)(ws + L.dshards);
   b.lists = (int32_t*)(ws + L.lists);
-  b.list_count = b.lists + 5 * L.shards_n;
+  b.list_count = b.lists + 6 * L.shards_n;
   b.o0tabs = (O0Tab*)(ws + L.o0tabs);
   b.sink = ws + L.sink;
   b.queues = (int32_t*)(ws + L.queues);

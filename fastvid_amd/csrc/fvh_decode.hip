@@ -27,11 +27,16 @@ __device__ __forceinline__ uint64_t rd_u64(const uint8_t* p) {
 // tier has its own shard list and decoder instance.
 constexpr int kO0Tiers = 3;
 __host__ __device__ constexpr int o0_tier_cap(int t) { return t == 0 ? 16 : t == 1 ? 64 : 256; }
+// decoder shard lists (and their counters): 0 order-0, 1 zero-run and
+// block-pack (and Rice shards handed back), 2.. order-0 tiers, then Rice
+constexpr int kRiceList = 2 + kO0Tiers;
+constexpr int kListCounts = kRiceList + 1;
+constexpr int kRiceQueue = 7;
 
 __global__ void k_dstate_init(FrameState* __restrict__ states, int frame_count,
                               int32_t* __restrict__ list_count, int32_t* __restrict__ queues) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 2 + kO0Tiers) list_count[i] = 0;  // order-0, other, one per order-0 tier
+  if (i < kListCounts) list_count[i] = 0;
   if (i < kQueueCount) queues[i] = 0;
   if (i >= frame_count) return;
   FrameState s = {};
@@ -178,12 +183,21 @@ __global__ __launch_bounds__(256) void k_parse(const FrameDesc* __restrict__ fra
     set_status(st, kStatusShardHeader);
     return;
   }
-  // file the tile's shards by decoder kernel
+  // file the tile's shards by decoder kernel: order-0, Rice, the rest
+  int n_rice = 0;
+  for (int ls = 0; ls < td.shard_count; ++ls) {
+    const int mode = dshards[td.first_shard + ls].mode;
+    n_rice += mode >= 1 && mode <= kModeRiceBase + kMaxRice;
+  }
+  const int n_other = td.shard_count - n_order0 - n_rice;
   int at0 = n_order0 ? atomicAdd(&list_count[0], n_order0) : 0;
-  int at1 = td.shard_count - n_order0 ? atomicAdd(&list_count[1], td.shard_count - n_order0) : 0;
+  int at1 = n_other ? atomicAdd(&list_count[1], n_other) : 0;
+  int at5 = n_rice ? atomicAdd(&list_count[kRiceList], n_rice) : 0;
   for (int ls = 0; ls < td.shard_count; ++ls) {
     const int shard = td.first_shard + ls;
-    if (dshards[shard].mode == kModeOrder0) lists[at0++] = shard;
+    const int mode = dshards[shard].mode;
+    if (mode == kModeOrder0) lists[at0++] = shard;
+    else if (mode >= 1 && mode <= kModeRiceBase + kMaxRice) lists[kRiceList * shard_total + at5++] = shard;
     else lists[shard_total + at1++] = shard;
   }
 }
@@ -961,6 +975,219 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
   }
 }
 
+// Rice shards (decode.cu:297-340): sixteen shards per wave, a quad of lanes
+// per shard, lane r decoding Rice lane r (symbols r, r + 4, ...), waves
+// persistent over the Rice list like k_dec_o0.  A lane keeps a 64-bit bit
+// buffer (next bit lowest) that one 4-byte refill per step holds at >= 32
+// bits, so a code of at most 32 bits (its zero run found by one ctz, then
+// the one, then k remainder bits) decodes in one step.  Refills take the
+// bytes at `pos` of the lane's two-chunk window A:B: a round of four steps
+// reads at most 16 bytes from pos < 16, so the window moves by one chunk at
+// most per round, through the same chunk pipeline as k_dec_o0 (C, D behind
+// it, the next chunk loaded as the round's last memory operation by lanes
+// that moved).  Decoded values are staged per shard in LDS and leave as one
+// 16-byte store per lane every two rounds.  A lane that meets a longer code
+// stops, and its shard goes back on the zero-run / block-pack list, where
+// k_dec_other decodes it bit by bit; bytes read past a lane's end fail the
+// final "consumed bits" check exactly where the reference runs out of bits.
+constexpr int kRiceShards = 16;
+
+// Four bytes starting at byte `pos` (0..27) of the 32-byte window
+// a0:a1:b0:b1, the first byte least significant.  (Taken by value: a select
+// among references would address the window through memory.)
+__device__ __forceinline__ uint32_t window4_le(uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1,
+                                               int pos) {
+  const int idx = pos >> 3;
+  const uint64_t lo = idx == 0 ? a0 : idx == 1 ? a1 : idx == 2 ? b0 : b1;
+  const uint64_t hi = idx == 0 ? a1 : idx == 1 ? b0 : b1;
+  const uint32_t sh = 8u * (uint32_t)(pos & 7);
+  return sh ? (uint32_t)((lo >> sh) | (hi << (64u - sh))) : (uint32_t)lo;
+}
+
+template <typename FT>
+__global__ __launch_bounds__(64) void k_dec_rice(const FrameDesc* __restrict__ frames,
+                                                 FrameState* __restrict__ states,
+                                                 const DShard* __restrict__ dshards,
+                                                 const int32_t* __restrict__ rice_list,
+                                                 const int32_t* __restrict__ rice_count,
+                                                 int32_t* __restrict__ other_list,
+                                                 int32_t* __restrict__ other_count,
+                                                 FT* __restrict__ folded, char* gbase,
+                                                 uint8_t* __restrict__ sink,
+                                                 int32_t* __restrict__ queue) {
+  __shared__ uint32_t stage_all[kRiceShards][O0Stage<FT>::kWords];
+  const int lane = lane_id();
+  const int g = lane >> 2;
+  const int r = lane & 3;
+  const int count = *rice_count;
+  FT* const stage = (FT*)stage_all[g];
+  for (int grp = blockIdx.x;;) {
+    if (grp * kRiceShards >= count) break;
+    const int at = grp * kRiceShards + g;
+    const int item = at < count ? rice_list[at] : -1;
+    DShard d = {};
+    if (item >= 0) d = dshards[item];
+    const FrameDesc& f = frames[d.frame];
+    FrameState* const st = &states[d.frame];
+    const uint32_t max_folded = (uint32_t)f.max_sample * 2u;
+    const uint8_t* body = as_global(gbase, (const uint8_t*)f.stream + d.body_offset);
+    const int n = item >= 0 ? d.count : 0;
+    const int k = item >= 0 ? d.mode - 1 : 0;
+    const int64_t body_length = d.body_length;
+    // lane table (decode.cu:301-313): lane lengths 0..2, the last lane takes
+    // the rest of the body
+    const int lane_count = n < 4 ? n : 4;
+    const int64_t table_bytes = (int64_t)(lane_count - 1) * 4;
+    const bool mine = r < lane_count;
+    bool ok = mine && table_bytes <= body_length;
+    int64_t lane_offset = table_bytes;
+    int64_t lane_length = 0;
+    if (ok) {
+      for (int i = 0; i < r; ++i) lane_offset += rd_u32(body + 4 * i);
+      lane_length = r + 1 < lane_count ? (int64_t)rd_u32(body + 4 * r) : body_length - lane_offset;
+      if (lane_offset > body_length || lane_length < 0 || lane_length > body_length - lane_offset) {
+        ok = false;
+      }
+    }
+    const int length = ok ? (int)lane_length : 0;
+    // aligned chunks holding the lane's bytes (a lane without bytes reads the
+    // body's first chunk and never uses it)
+    const uint8_t* first = length > 0 ? body + lane_offset : body;
+    const int head = (int)((uintptr_t)first & 15);
+    const uint4* cp = (const uint4*)(first - head);
+    const int last = length > 0 ? (head + length - 1) >> 4 : 0;
+    const Chunk A0 = load_chunk(cp);
+    const Chunk B0 = load_chunk(cp + min(1, last));
+    const Chunk C0 = load_chunk(cp + min(2, last));
+    const Chunk D0 = load_chunk(cp + min(3, last));
+    uint64_t a0 = A0.lo, a1 = A0.hi, b0 = B0.lo, b1 = B0.hi, c0 = C0.lo, c1 = C0.hi;
+    uint64_t d0 = D0.lo, d1 = D0.hi;  // chunk ci + 3, or in flight into n0:n1
+    uint64_t n0 = 0, n1 = 0;
+    bool pend = false;  // n0:n1 holds chunk ci + 3 once its load lands
+    int ci = 0;         // chunk index of A
+    int pos = head;     // next byte within A:B, < 16 when a round starts
+    uint64_t buf = 0;   // bit buffer, next bit lowest
+    int nb = 0;         // bits in buf
+    int consumed = 0;   // lane bits consumed
+    bool slow = false;  // met a code longer than 32 bits
+    const uint32_t kmask = (1u << k) - 1u;
+    FT* const out = folded + d.folded_offset;
+    const int nsteps = ok ? (n - r + 3) >> 2 : 0;
+    const int wave_steps = __builtin_amdgcn_readfirstlane((int)wave_max((uint32_t)nsteps));
+    const int full_steps = min(wave_steps, __builtin_amdgcn_readfirstlane((int)wave_min(
+                                               (uint32_t)(item >= 0 && mine ? nsteps : 0x7fffffff))));
+    // lanes without symbols store to their sink slot
+    FT* const flush_base = nsteps > 0 ? out : (FT*)(sink + lane * kLaneSinkBytes);
+    const int flush_step = nsteps > 0 ? 1 : 0;
+    __builtin_amdgcn_s_waitcnt(0);
+
+    auto step = [&](int slot_pos, bool live) __attribute__((always_inline)) {
+      // refill: the four bytes at pos of A:B (pos < 28 within a round)
+      const bool fill = nb < 32;
+      const uint32_t word = window4_le(a0, a1, b0, b1, pos);
+      buf = fill ? buf | ((uint64_t)word << nb) : buf;
+      pos += fill ? 4 : 0;
+      nb += fill ? 32 : 0;
+      // one code: z zero bits, a one, k remainder bits
+      const uint32_t low = (uint32_t)buf;
+      const int z = low ? (int)__builtin_ctz(low) : 32;
+      const int used = z + 1 + k;
+      const bool fits = used <= 32;
+      const uint32_t value = ((uint32_t)z << k) | ((low >> ((z + 1) & 31)) & kmask);
+      const bool take = live && fits && !slow;
+      if (live && !fits) slow = true;
+      stage[4 * slot_pos + r] = (FT)value;
+      ok = ok && !(take && value > max_folded);
+      const int u = take ? used : 0;
+      buf >>= u;
+      nb -= u;
+      consumed += u;
+    };
+    // the 32 symbols of two rounds: lane r stores symbols 8 r .. 8 r + 7
+    auto flush_full = [&](int s0) __attribute__((always_inline)) {
+      FT* const dst = flush_base + (4 * s0 + 8 * r) * flush_step;
+      if constexpr (sizeof(FT) == 2) {
+        *(uint4*)dst = *(const uint4*)(stage + 8 * r);
+      } else {
+        ((uint4*)dst)[0] = ((const uint4*)(stage + 8 * r))[0];
+        ((uint4*)dst)[1] = ((const uint4*)(stage + 8 * r))[1];
+      }
+    };
+    auto flush = [&](int s0) __attribute__((always_inline)) {
+      const int p0 = 4 * s0 + 8 * r;
+      if (p0 + 8 <= n) {
+        flush_full(s0);
+      } else {
+        for (int j = 0; p0 + j < n; ++j) out[p0 + j] = stage[8 * r + j];
+      }
+    };
+    // one round: four steps, then the window moves by one chunk at most
+    auto round = [&](int s0, int half, bool predicated) __attribute__((always_inline)) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        step(4 * half + t, !predicated || s0 + 4 * half + t < nsteps);
+      }
+      const bool adv = pos >= 16;
+      // (kept after the steps: the wait for last round's load goes here)
+      __builtin_amdgcn_sched_barrier(0);
+      d0 = pend ? n0 : d0;
+      d1 = pend ? n1 : d1;
+      a0 = adv ? b0 : a0;
+      a1 = adv ? b1 : a1;
+      b0 = adv ? c0 : b0;
+      b1 = adv ? c1 : b1;
+      c0 = adv ? d0 : c0;
+      c1 = adv ? d1 : c1;
+      pos -= adv ? 16 : 0;
+      ci += adv ? 1 : 0;
+      pend = adv;
+    };
+    // lanes that did not move read one line every such lane of the wave
+    // shares, so the load stays unconditional and n0:n1 needs no merge
+    auto refill = [&]() __attribute__((always_inline)) {
+      __builtin_amdgcn_sched_barrier(0);
+      const Chunk Nn = load_chunk(pend ? cp + min(ci + 3, last) : (const uint4*)dshards);
+      n0 = Nn.lo;
+      n1 = Nn.hi;
+      __builtin_amdgcn_sched_barrier(0);
+    };
+
+    int s0 = 0;
+    for (; s0 + 8 <= full_steps; s0 += 8) {
+      round(s0, 0, false);
+      refill();
+      round(s0, 1, false);
+      flush_full(s0);
+      refill();
+    }
+    for (; s0 < wave_steps; s0 += 8) {
+      round(s0, 0, true);
+      refill();
+      round(s0, 1, true);
+      flush(s0);
+      refill();
+    }
+    // end of the lane (decode.cu:328-338): every byte used, zero padding bits
+    if (ok && !slow) {
+      const int used_bytes = (consumed + 7) >> 3;
+      if (used_bytes != length) {
+        ok = false;
+      } else if ((consumed & 7) != 0) {
+        const uint32_t pad = 8u - (uint32_t)(consumed & 7);
+        if ((buf & ((1u << pad) - 1u)) != 0) ok = false;
+      }
+    }
+    // a shard with a lane that stopped goes to k_dec_other whole
+    const uint64_t redo_lanes = ballot(item >= 0 && slow);
+    const uint64_t redo = redo_lanes & (0xfull << (4 * g));
+    if (item >= 0 && mine && !ok && redo == 0) set_status(st, 2);
+    if (r == 0 && redo != 0) other_list[atomicAdd(other_count, 1)] = item;
+    int next = 0;
+    if (lane == 0) next = atomicAdd(queue, 1);
+    grp = (int)gridDim.x + __builtin_amdgcn_readfirstlane(next);
+  }
+}
+
 // predictor="serial": one thread per tile in raster order (decode.cu:608-656).
 template <typename FT>
 __global__ __launch_bounds__(256) void k_serial(const FrameDesc* __restrict__ frames,
@@ -1044,6 +1271,14 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
     hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(2) - 1>), grid, dim3(64), 0, stream, b.frames,
                        b.dshards, b.lists, tiers + 2 * b.shard_total, b.list_count + 4, b.o0tabs,
                        b.states, (FT*)b.folded, (char*)b.folded, b.sink, b.queues + 6);
+  }
+  {
+    FVH_TIMED("k_dec_rice", stream);
+    const dim3 grid(min((b.shard_total + kRiceShards - 1) / kRiceShards, 16 * device_cus()));
+    hipLaunchKernelGGL((k_dec_rice<FT>), grid, dim3(64), 0, stream, b.frames, b.states, b.dshards,
+                       b.lists + kRiceList * b.shard_total, b.list_count + kRiceList,
+                       b.lists + b.shard_total, b.list_count + 1, (FT*)b.folded, (char*)b.folded,
+                       b.sink, b.queues + kRiceQueue);
   }
   {
     FVH_TIMED("k_dec_other", stream);

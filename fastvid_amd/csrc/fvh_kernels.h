@@ -53,9 +53,10 @@ struct DecodeBuffers {
   TileDesc* tiles;
   ShardDesc* shards;
   DShard* dshards;
-  int32_t* lists;        // [0, n): order-0 shards, [n, 2n): others, [(2+t) n, (3+t) n):
-                         // order-0 list positions of alphabet tier t (n = shard_total)
-  int32_t* list_count;   // [0] order-0, [1] others, [2 + t] tier t
+  int32_t* lists;        // [0, n): order-0 shards, [n, 2n): zero-run / block-pack,
+                         // [(2+t) n, (3+t) n): order-0 list positions of alphabet
+                         // tier t, [5n, 6n): Rice shards (n = shard_total)
+  int32_t* list_count;   // [0] order-0, [1] zero-run / block-pack, [2 + t] tier t, [5] Rice
   O0Tab* o0tabs;         // parsed order-0 tables, by order-0 list position
   uint8_t* sink;         // kSinkBytes target of masked-off stores
   int32_t* queues;       // kQueueCount work counters, zeroed by k_dstate_init
