@@ -414,10 +414,125 @@ __device__ __forceinline__ void dec_other_body(const uint8_t* body, int64_t body
   }
 }
 
-// Zero-run, Rice and block-pack shards (decode.cu:222-399): one wave per
-// shard, shards taken from the "other" list by a grid-stride loop.  The body
-// is first copied to LDS so the serial token walks and block-offset walks read
-// LDS instead of issuing dependent global loads.
+// Zero-run and block-pack bodies staged in LDS (`bl`, length L <= 8960),
+// decoded by the whole wave; `stage` is an n-symbol LDS image of the shard.
+//
+// Zero-run (decode.cu:237-295): a byte with its high bit clear ends a token,
+// so every token is found at once; its lane reads back the (at most four)
+// bytes before it and checks the varint as the reference does (at most five
+// bytes, the fifth <= 0x0f, canonical).  Tokens place their symbols by a
+// wave scan of their counts (a run, or one value) in a zero-filled image.
+// The reference's sequential walk succeeds exactly when every token is
+// valid, the counts sum to n and the body ends on a token's end, which is
+// what is checked.
+// Block-pack (decode.cu:342-396): lane b finds block b's offset by walking
+// the widths before it and checks the block; then every lane extracts
+// values from aligned dword pairs.
+template <typename FT>
+__device__ __forceinline__ void dec_other_wave(const uint8_t* bl, int head, int L, int n,
+                                               FT* __restrict__ out, int mode,
+                                               uint32_t max_folded, FrameState* st, int lane,
+                                               FT* stage, uint2* blocks) {
+  constexpr int kPer = 16 / (int)sizeof(FT);  // symbols per 16-byte chunk
+  if (mode == 0) {
+    const int chunks = (n + kPer - 1) / kPer;
+    for (int i = lane; i < chunks * kPer; i += 64) stage[i] = 0;
+    __syncthreads();
+    bool bad = L == 0 || (bl[L - 1] & 0x80) != 0;  // no tokens, or the last one unfinished
+    int carry = 0;  // symbols placed by earlier passes (capped past n)
+    for (int base = 0; base < L; base += 64) {
+      const int i = base + lane;
+      const uint32_t byte = i < L ? bl[i] : 0x80u;
+      const bool end = (byte & 0x80u) == 0;
+      uint32_t token = 0;
+      int count = 0;
+      if (end) {
+        // the token's first byte: back over at most four continuation bytes
+        int j = i;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (j > 0 && (bl[j - 1] & 0x80u)) --j;
+        }
+        const int used = i - j + 1;
+        if (j > 0 && (bl[j - 1] & 0x80u)) bad = true;  // a sixth byte
+        for (int q = 0; q < used; ++q) token |= (uint32_t)(bl[j + q] & 0x7fu) << (7 * q);
+        if (used == 5 && bl[j + 4] > 0x0fu) bad = true;
+        if (used != varint_len(token)) bad = true;
+        count = (token & 1u) ? 1 : (int)min(token / 2u + 1u, (uint32_t)n + 1u);
+      }
+      const int incl = wave_incl_scan(count);
+      const int pos = carry + incl - count;
+      if (end && (token & 1u)) {
+        const uint32_t value = (token + 1u) / 2u;
+        if (value > max_folded) bad = true;
+        if (pos < n) stage[pos] = (FT)value;
+      }
+      carry = min(carry + __builtin_amdgcn_readlane(incl, 63), n + 1);
+    }
+    if (carry != n) bad = true;
+    if (ballot(bad) != 0) {
+      if (lane == 0) set_status(st, 1);
+      return;
+    }
+    __syncthreads();
+    // whole 16-byte chunks of the image (a tile's residual area is padded to
+    // a multiple of 8 symbols, so the last chunk stays inside it)
+    for (int c = lane; c < chunks; c += 64) ((uint4*)out)[c] = ((const uint4*)stage)[c];
+  } else {  // block-pack
+    const int block_count = (n + kBlockSymbols - 1) / kBlockSymbols;
+    bool bad = false;
+    if (lane < block_count) {
+      int cursor = 0;
+      for (int prior = 0; !bad && prior < lane; ++prior) {
+        if (cursor >= L) {
+          bad = true;
+          break;
+        }
+        const int w = bl[cursor++];
+        const int pr = n - prior * kBlockSymbols;
+        cursor += ((pr < kBlockSymbols ? pr : kBlockSymbols) * w + 7) / 8;
+      }
+      if (!bad && cursor >= L) bad = true;
+      const int w = bad ? 0 : bl[cursor++];
+      if (w > bit_length(max_folded)) bad = true;
+      const int rem = n - lane * kBlockSymbols;
+      const int cnt = rem < kBlockSymbols ? rem : kBlockSymbols;
+      const int payload = (cnt * w + 7) / 8;
+      if (!bad && payload > L - cursor) bad = true;
+      if (!bad && lane + 1 == block_count && cursor + payload != L) bad = true;
+      if (!bad && ((cnt * w) & 7) != 0) {
+        const uint32_t mask = ~((1u << ((cnt * w) & 7)) - 1u) & 0xffu;
+        if ((bl[cursor + payload - 1] & mask) != 0) bad = true;
+      }
+      blocks[lane] = make_uint2((uint32_t)cursor, bad ? 0u : (uint32_t)w);
+    }
+    if (ballot(bad) != 0) {
+      if (lane == 0) set_status(st, 3);
+      return;
+    }
+    __syncthreads();
+    const uint32_t* words = (const uint32_t*)(bl - head);  // the staged, aligned chunks
+    for (int i = lane; i < n; i += 64) {
+      const uint2 blk = blocks[i / kBlockSymbols];
+      const uint32_t bit = (uint32_t)(head + (int)blk.x) * 8u + (uint32_t)(i % kBlockSymbols) * blk.y;
+      const uint32_t wi = bit >> 5;
+      const uint64_t pair = ((uint64_t)words[wi + 1] << 32) | words[wi];
+      const uint32_t value = blk.y ? (uint32_t)(pair >> (bit & 31u)) & ((1u << blk.y) - 1u) : 0u;
+      if (value > max_folded) bad = true;
+      out[i] = (FT)value;
+    }
+    if (ballot(bad) != 0 && lane == 0) set_status(st, 3);
+  }
+}
+
+// Zero-run and block-pack shards, and Rice shards k_dec_rice handed back
+// (decode.cu:222-399): one wave per shard, shards taken from the list by a
+// grid-stride loop.  The body is first copied to LDS; zero-run and
+// block-pack bodies are then decoded by the whole wave (dec_other_wave), Rice
+// bodies by one lane per Rice lane, and bodies too long to stage (malformed)
+// symbol by symbol from global memory.
+constexpr int kBodyLds = 65536 / 8 + 768 + 16;  // staged body bytes (+ head)
+
 template <typename FT>
 __global__ __launch_bounds__(64) void k_dec_other(const FrameDesc* __restrict__ frames,
                                                   FrameState* __restrict__ states,
@@ -426,7 +541,10 @@ __global__ __launch_bounds__(64) void k_dec_other(const FrameDesc* __restrict__ 
                                                   const int32_t* __restrict__ list_count,
                                                   int shard_total, FT* __restrict__ folded,
                                                   char* gbase) {
-  __shared__ __attribute__((aligned(16))) uint8_t body_lds[65536 / 8 + 768 + 16];
+  // (16 bytes of slack: block-pack reads the dword after a value's)
+  __shared__ __attribute__((aligned(16))) uint8_t body_lds[kBodyLds + 16];
+  __shared__ __attribute__((aligned(16))) FT stage[kShardSymbols];
+  __shared__ uint2 blocks[kShardSymbols / kBlockSymbols];
   const int lane = lane_id();
   const int total = list_count[1];
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
@@ -447,14 +565,17 @@ __global__ __launch_bounds__(64) void k_dec_other(const FrameDesc* __restrict__ 
     // crosses a page, so the bytes before and after the body it covers are
     // readable whenever the body is.
     const int head = (int)((uintptr_t)gbody & 15);
-    const bool staged = body_length + head <= (int64_t)sizeof(body_lds);
+    const bool staged = body_length + head <= (int64_t)kBodyLds;
     if (staged) {
       const uint4* chunks = (const uint4*)(gbody - head);
       const int nchunks = body_length > 0 ? (int)((head + body_length + 15) >> 4) : 0;
       for (int k = lane; k < nchunks; k += 64) ((uint4*)body_lds)[k] = chunks[k];
     }
     __syncthreads();
-    if (staged) {
+    if (staged && (mode == 0 || mode == kModeBlockPack)) {
+      dec_other_wave(body_lds + head, head, (int)body_length, (int)n, out, mode, max_folded, st,
+                     lane, stage, blocks);
+    } else if (staged) {
       dec_other_body(body_lds + head, body_length, n, out, mode, max_folded, st, lane);
     } else {
       dec_other_body(gbody, body_length, n, out, mode, max_folded, st, lane);
@@ -986,10 +1107,14 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
 // most per round, through the same chunk pipeline as k_dec_o0 (C, D behind
 // it, the next chunk loaded as the round's last memory operation by lanes
 // that moved).  Decoded values are staged per shard in LDS and leave as one
-// 16-byte store per lane every two rounds.  A lane that meets a longer code
-// stops, and its shard goes back on the zero-run / block-pack list, where
-// k_dec_other decodes it bit by bit; bytes read past a lane's end fail the
-// final "consumed bits" check exactly where the reference runs out of bits.
+// 16-byte store per lane every two rounds.  A longer code (a zero run past
+// the buffer) is read from the lane's chunks in memory and the window is
+// reloaded where it ends; after that the round's remaining refills still
+// read inside A:B (pos <= 19 + 8).  A lane whose stream the reference
+// rejects inside such a code stops, and its shard goes back on the zero-run
+// / block-pack list, where k_dec_other decodes it bit by bit and reports it;
+// bytes read past a lane's end fail the final "consumed bits" check exactly
+// where the reference runs out of bits.
 constexpr int kRiceShards = 16;
 
 // Four bytes starting at byte `pos` (0..27) of the 32-byte window
@@ -1002,6 +1127,15 @@ __device__ __forceinline__ uint32_t window4_le(uint64_t a0, uint64_t a1, uint64_
   const uint64_t hi = idx == 0 ? a1 : idx == 1 ? b0 : b1;
   const uint32_t sh = 8u * (uint32_t)(pos & 7);
   return sh ? (uint32_t)((lo >> sh) | (hi << (64u - sh))) : (uint32_t)lo;
+}
+
+// Eight bytes starting at byte `o` (0..15) of two consecutive 16-byte chunks,
+// the first byte least significant.
+__device__ __forceinline__ uint64_t bytes8_le(uint64_t x0, uint64_t x1, uint64_t y0, int o) {
+  const uint64_t lo = o < 8 ? x0 : x1;
+  const uint64_t hi = o < 8 ? x1 : y0;
+  const uint32_t sh = 8u * (uint32_t)(o & 7);
+  return sh ? (lo >> sh) | (hi << (64u - sh)) : lo;
 }
 
 template <typename FT>
@@ -1093,9 +1227,66 @@ __global__ __launch_bounds__(64) void k_dec_rice(const FrameDesc* __restrict__ f
       const int z = low ? (int)__builtin_ctz(low) : 32;
       const int used = z + 1 + k;
       const bool fits = used <= 32;
-      const uint32_t value = ((uint32_t)z << k) | ((low >> ((z + 1) & 31)) & kmask);
+      uint32_t value = ((uint32_t)z << k) | ((low >> ((z + 1) & 31)) & kmask);
       const bool take = live && fits && !slow;
-      if (live && !fits) slow = true;
+      if (live && !fits && !slow) {
+        // a longer code: read it from the lane's aligned chunks in memory
+        // (a chunk never crosses a page, so bytes past the lane's end are
+        // readable and only fail the final check), then reload the window
+        // where it ends
+        auto bits_at = [&](int bit) __attribute__((always_inline)) {
+          const int a = head + (bit >> 3);
+          const Chunk x = load_chunk(cp + min(a >> 4, last));
+          const Chunk y = load_chunk(cp + min((a >> 4) + 1, last));
+          return bytes8_le(x.lo, x.hi, y.lo, a & 15) >> (bit & 7);
+        };
+        int bit = consumed;
+        uint64_t q = 0;
+        bool good = true;
+        for (;;) {
+          if ((bit >> 3) >= length || q > max_folded) {
+            good = false;
+            break;
+          }
+          const uint64_t w = bits_at(bit);
+          const int avail = 64 - (bit & 7);
+          if (w == 0) {
+            q += (uint64_t)avail;
+            bit += avail;
+            continue;
+          }
+          const int zz = (int)__builtin_ctzll(w);
+          q += (uint64_t)zz;
+          bit += zz + 1;
+          break;
+        }
+        uint64_t v = q << k;
+        if (good && k > 0) {
+          v |= bits_at(bit) & kmask;
+          bit += k;
+        }
+        if (!good || q > max_folded || v > max_folded) {
+          slow = true;  // the reference fails here: k_dec_other reports it
+        } else {
+          value = (uint32_t)v;
+          consumed = bit;
+          const int a = head + (bit >> 3);
+          ci = a >> 4;
+          pos = a & 15;
+          const Chunk A1 = load_chunk(cp + min(ci, last));
+          const Chunk B1 = load_chunk(cp + min(ci + 1, last));
+          const Chunk C1 = load_chunk(cp + min(ci + 2, last));
+          const Chunk D1 = load_chunk(cp + min(ci + 3, last));
+          a0 = A1.lo, a1 = A1.hi, b0 = B1.lo, b1 = B1.hi;
+          c0 = C1.lo, c1 = C1.hi, d0 = D1.lo, d1 = D1.hi;
+          pend = false;  // the chunk in flight belongs to the old window
+          buf = (uint64_t)(window4_le(a0, a1, b0, b1, pos) >> (bit & 7));
+          nb = 32 - (bit & 7);
+          pos += 4;
+        }
+        // nothing of this branch stays in flight past it
+        __builtin_amdgcn_s_waitcnt(0);
+      }
       stage[4 * slot_pos + r] = (FT)value;
       ok = ok && !(take && value > max_folded);
       const int u = take ? used : 0;

@@ -168,6 +168,27 @@ def test_wide_alphabet_shards_match_oracle(fc, oracle, layout, width, height, ti
         assert np.array_equal(_host(a), b)
 
 
+def test_long_rice_codes_match_oracle(fc, oracle):
+    """10-bit lossless noise with sparse spikes: the shards pick Rice with
+    k = 3 or 4 and the spikes give codes of about seventy bits, longer than
+    the sixteen-shards-per-wave Rice decoder takes, so those shards are
+    decoded again bit by bit; streams and round trip must still match."""
+    rng = np.random.default_rng(11)
+    base = 512 + rng.normal(0, 1.5, size=(96, 160))
+    spikes = rng.random(base.shape) < 0.004
+    base[spikes] = rng.choice([0, 1000], size=int(spikes.sum()))
+    plane = np.clip(np.rint(base), 0, 1023).astype(np.uint16)
+    stream = fc.encode(_cuda((plane,)), layout="gray", bit_depth=10, quality=100,
+                       tile_size=(64, 64))
+    expected = oracle.encode((plane,), layout="gray", bit_depth=10, quality=100,
+                             tile_size=(64, 64))
+    got = _bytes(stream)
+    assert got == expected
+    assert shard_modes(got) & set(range(1, 18))
+    (decoded,) = fc.decode(stream)
+    assert np.array_equal(_host(decoded), plane)
+
+
 def test_out_of_range_sample_fails_like_reference(fc):
     plane = np.full((20, 30), 300, np.uint16)
     with pytest.raises(RuntimeError, match="status 1"):
