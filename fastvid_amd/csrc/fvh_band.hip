@@ -203,7 +203,7 @@ struct BandTypes {
 template <bool kEncode, typename FT>
 __host__ __device__ constexpr int band_lds_bytes(int ring_groups) {
   return (BandTypes<kEncode, FT>::kLineQuads + BandTypes<kEncode, FT>::kOutQuads) * 16 +
-         4 * ring_groups * 32;
+         4 * (2 * ring_groups + 1) * 16;
 }
 
 // Memory traffic goes through per-row line buffers in LDS.  A "line" is 64
@@ -283,7 +283,8 @@ __device__ __forceinline__ void band_group(
   v4u* const lin = smem + grp * (kBandRows * kRowChunks * kQi);
   v4u* const lout =
       T::kInPlace ? lin : smem + T::kLineQuads + grp * (kBandRows * kRowChunks * kQo);
-  v4u* const ring = smem + T::kLineQuads + T::kOutQuads + grp * ring_groups * 2;
+  // (rings one quad apart: the four tiles' slots fall on different banks)
+  v4u* const ring = smem + T::kLineQuads + T::kOutQuads + grp * (ring_groups * 2 + 1);
   v4u* const lin_w = lin + ((8 * half) * kRowChunks + c) * kQi;          // + k rows
   const v4u* const lout_r = lout + ((8 * half) * kRowChunks + c) * kQo;  // + ((k+1)&7) rows
   const v4u* const lin_row = lin + i * kRowChunks * kQi;
@@ -476,7 +477,10 @@ __device__ __forceinline__ void band_group(
       ++bo;
     }
     {
-      const v4u* rq = ring + max(cg, 0) * 2;
+      // only lane 0 of a tile uses the slot: every lane reads lane 0's, one
+      // address per tile (lane i runs i groups behind lane 0)
+      const int cg0 = cg + i >= GP ? cg + i - GP : cg + i;
+      const v4u* rq = ring + cg0 * 2;
       rv0 = rq[0];
       rv1 = rq[1];
     }

@@ -1133,10 +1133,10 @@ __global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frame
                                              const uint8_t* __restrict__ rank_area,
                                              int cap_below) {
   (void)folded;
-  // one entry of padding per row: the shards' rows start four banks apart,
-  // so the sixteen shards' lookups of their commonest ranks do not all meet
-  // on the same banks
-  __shared__ uint4 ent_all[kRansShards][kCap + 1];
+  // two entries of padding per row: the shards' rows start eight banks
+  // apart, so the four shards a 16-lane pass of a 16-byte LDS read covers
+  // take their two commonest ranks from distinct banks
+  __shared__ uint4 ent_all[kRansShards][kCap + 2];
   const int groups = (shard_total + kRansShards - 1) / kRansShards;
   for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
     rans_group<FT>(frames, states, tiles, shards, shard_total, recs, scratch, rank_area,
