@@ -21,10 +21,11 @@ import time
 from dataclasses import dataclass
 from typing import Sequence
 
+import numpy as np
 import torch
 
 from . import _native
-from .codec import EncodeJob, _read_header, frame_args
+from .codec import EncodeJob, _read_header, _read_results, frame_args
 
 
 @dataclass(frozen=True)
@@ -162,8 +163,11 @@ class GraphDecoder:
                                         device=self.device) for i in range(n)]
             self.results = torch.empty(n * _native.RESULT_BYTES, dtype=torch.uint8,
                                        device=self.device)
+            self._host_results = torch.empty(n * _native.RESULT_BYTES, dtype=torch.uint8,
+                                             pin_memory=True)
             self._ins = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.streams])
             self._outs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.outputs])
+            self._frames = [self.planes(i) for i in range(n)]
             self.wavefront = bool(wavefront)
             self.stream = torch.cuda.Stream(self.device)
             self.graph = _capture(self._launch_now, self.stream) if graph else None
@@ -214,17 +218,14 @@ class GraphDecoder:
         return tuple(planes)
 
     def finish(self) -> list:
-        self.stream.synchronize()
-        host = self.results.cpu().numpy().tobytes()
-        frames = []
-        for i in range(len(self.streams)):
-            r = _native.Result.from_buffer_copy(host, i * _native.RESULT_BYTES)
-            if r.status == 12:
+        status = _read_results(self.results, self._host_results, self.stream)["status"]
+        bad = np.flatnonzero(status)
+        if bad.size:
+            first = int(status[bad[0]])
+            if first == 12:
                 raise RuntimeError("stream header does not match the planned geometry")
-            if r.status != 0:
-                raise RuntimeError(f"malformed entropy payload (CUDA status {r.status})")
-            frames.append(self.planes(i))
-        return frames
+            raise RuntimeError(f"malformed entropy payload (CUDA status {first})")
+        return list(self._frames)
 
     def decode(self, streams: Sequence[torch.Tensor]) -> list:
         self.load(streams)

@@ -15,9 +15,25 @@ import ctypes
 from dataclasses import dataclass
 from typing import Sequence
 
+import numpy as np
 import torch
 
 from . import _native
+
+# Per-frame result record written by the last kernel of a job (C struct FvhResult).
+_RESULT_DTYPE = np.dtype([("stream_length", "<i8"), ("status", "<i4"), ("flags", "<i4")])
+assert _RESULT_DTYPE.itemsize == _native.RESULT_BYTES
+
+
+def _read_results(results: torch.Tensor, host: torch.Tensor, stream) -> np.ndarray:
+    """Copies the result records into pinned host memory on the stream the job
+    ran on and waits for that stream only."""
+    if stream is None:
+        stream = torch.cuda.current_stream(results.device)
+    with torch.cuda.stream(stream):
+        host.copy_(results, non_blocking=True)
+    stream.synchronize()
+    return host.numpy().view(_RESULT_DTYPE)
 
 _LAYOUTS = {"gray": 0, "yuv422": 1, "rgb444": 2}
 
@@ -150,6 +166,7 @@ class EncodeJob:
         self.workspace = None
         self.output = None
         self.results = None
+        self._stream = None
 
     def allocate(self, workspace=None, output=None, results=None):
         n = len(self.frames)
@@ -159,6 +176,8 @@ class EncodeJob:
             max(self.output_bytes, 1), dtype=torch.uint8, device=self.device)
         self.results = results if results is not None else torch.empty(
             n * _native.RESULT_BYTES, dtype=torch.uint8, device=self.device)
+        self._host_results = torch.empty(n * _native.RESULT_BYTES, dtype=torch.uint8,
+                                         pin_memory=True)
         return self
 
     def launch(self, stream=None):
@@ -168,6 +187,7 @@ class EncodeJob:
         msg = _native.message_buffer()
         with torch.cuda.device(self.device):
             stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+            self._stream = stream
             _native.check(lib.fvh_encode_launch(
                 ctypes.addressof(self._structs), len(self.frames), self.workspace.data_ptr(),
                 self.workspace.numel(), self.output.data_ptr(), self.output.numel(),
@@ -175,18 +195,29 @@ class EncodeJob:
         return self
 
     def finish(self):
-        host = self.results.cpu().numpy().tobytes()
-        streams = []
-        for i in range(len(self.frames)):
-            r = _native.Result.from_buffer_copy(host, i * _native.RESULT_BYTES)
-            if r.status != 0:
-                if r.status in (1, 2):
-                    raise RuntimeError(f"CUDA encoder analysis failed (status {r.status})")
-                if r.status == 100:
-                    raise RuntimeError("tile payload is too large")
-                raise RuntimeError(f"encoder failed (status {r.status})")
-            streams.append(self.output.narrow(0, int(self.offsets[i]), int(r.stream_length)))
-        return streams
+        rec = _read_results(self.results, self._host_results, self._stream)
+        bad = np.flatnonzero(rec["status"])
+        if bad.size:
+            status = int(rec["status"][bad[0]])
+            if status in (1, 2):
+                raise RuntimeError(f"CUDA encoder analysis failed (status {status})")
+            if status == 100:
+                raise RuntimeError("tile payload is too large")
+            raise RuntimeError(f"encoder failed (status {status})")
+        # Each stream starts at its frame's reserved offset: one split of the output
+        # into [gap, stream, gap, stream, ..., tail] yields every stream view at once.
+        starts = np.frombuffer(self.offsets, dtype=np.uint64).astype(np.int64)
+        lengths = rec["stream_length"]
+        ends = starts + lengths
+        gaps = starts - np.concatenate(([0], ends[:-1]))
+        tail = self.output.numel() - int(ends[-1])
+        if (gaps < 0).any() or tail < 0:
+            return [self.output.narrow(0, int(o), int(n)) for o, n in zip(starts, lengths)]
+        sizes = np.empty(2 * len(starts) + 1, dtype=np.int64)
+        sizes[0:-1:2] = gaps
+        sizes[1::2] = lengths
+        sizes[-1] = tail
+        return list(self.output.split_with_sizes(sizes.tolist())[1::2])
 
 
 def encode_batch(frames: Sequence[FrameArgs]) -> list:
@@ -263,6 +294,7 @@ class DecodeJob:
         self.workspace = None
         self.outputs = None
         self.results = None
+        self._stream = None
 
     def allocate(self, workspace=None, outputs=None, results=None):
         n = len(self.inputs)
@@ -273,6 +305,12 @@ class DecodeJob:
             for i in range(n)]
         self.results = results if results is not None else torch.empty(
             n * _native.RESULT_BYTES, dtype=torch.uint8, device=self.device)
+        self._host_results = torch.empty(n * _native.RESULT_BYTES, dtype=torch.uint8,
+                                         pin_memory=True)
+        self._ins = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.inputs])
+        self._outs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.outputs])
+        # The plane views of every output are fixed once the outputs are.
+        self._frames = [self.planes(i) for i in range(n)]
         return self
 
     def launch(self, stream=None):
@@ -281,12 +319,12 @@ class DecodeJob:
         n = len(self.inputs)
         lib = _native.lib()
         msg = _native.message_buffer()
-        ins = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.inputs])
-        outs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.outputs])
         with torch.cuda.device(self.device):
             stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+            self._stream = stream
             _native.check(lib.fvh_decode_launch(
-                ctypes.addressof(self.infos), ins, outs, n, self.workspace.data_ptr(),
+                ctypes.addressof(self.infos), self._ins, self._outs, n,
+                self.workspace.data_ptr(),
                 self.workspace.numel(), self.results.data_ptr(), int(self.wavefront),
                 ctypes.c_void_p(stream.cuda_stream), msg, len(msg)), msg)
         return self
@@ -304,14 +342,12 @@ class DecodeJob:
         return tuple(planes)
 
     def finish(self):
-        host = self.results.cpu().numpy().tobytes()
-        frames = []
-        for i in range(len(self.inputs)):
-            r = _native.Result.from_buffer_copy(host, i * _native.RESULT_BYTES)
-            if r.status != 0:
-                raise RuntimeError(f"malformed entropy payload (CUDA status {r.status})")
-            frames.append(self.planes(i))
-        return frames
+        rec = _read_results(self.results, self._host_results, self._stream)
+        bad = np.flatnonzero(rec["status"])
+        if bad.size:
+            raise RuntimeError(
+                f"malformed entropy payload (CUDA status {int(rec['status'][bad[0]])})")
+        return list(self._frames)
 
 
 def decode_batch(streams: Sequence[torch.Tensor], wavefront: bool = True) -> list:
