@@ -41,7 +41,15 @@
 namespace fvh {
 
 constexpr int kBandRows = 16;
-constexpr int kRowChunks = 9;              // line buffer row: 8 chunks + 1 against bank conflicts
+constexpr int kRowChunks = 8;              // chunks of 8 samples in one line buffer row
+// Line buffer of a tile: rows 0-7, one empty row, rows 8-15 (row r at row
+// slot r + r / 8).  Lane i reads and writes chunk (k - i) mod 8 of its own
+// row r = i in group k, which puts the 16 lanes of a tile on 16 different
+// 16-byte bank slots of a 64-bank LDS (and each half on 8 of a 32-bank one);
+// the line loads and stores (lane: row 8 * half + k', chunk c) stay on
+// consecutive slots.  A padded row of 9 chunks put every lane of a half on
+// the same slot: 8-way conflicts on 2 of the 4 LDS operations per group.
+constexpr int kLineRowSlots = kBandRows + 1;
 constexpr uint32_t kOob = 0x80000000u;     // buffer offset past every resource: access dropped
 constexpr int kRsrcWord3 = 0x00020000;     // raw buffer, 32-bit dwords (gfx9 family)
 constexpr int kAuxNt = 2;                  // buffer instruction cache policy: non-temporal
@@ -182,11 +190,57 @@ __device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) 
   return r;
 }
 
+__device__ __forceinline__ int mad_i24(int a, int b, int c) {
+  int r;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// a + b as one instruction the compiler cannot reassociate: the predictor's
+// gradient L + (A - UL) keeps A - UL off the reconstruction chain.
+__device__ __forceinline__ int add_asm(int a, int b) {
+  int r;
+  asm("v_add_u32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 template <typename FT, bool kOdd>
 __device__ __forceinline__ int quantize(int n, const Quant& q) {
   if (!kOdd) n -= (uint32_t)n < (uint32_t)q.offp ? 1 : 0;
   if (sizeof(FT) == 2) return (int)mad_u24((uint32_t)n, q.m, q.negks) >> q.s;
   return (int)__umulhi((uint32_t)n, q.m) - q.K;
+}
+
+// x * m + c for the low or high 16-bit sample of a word (m <= 2^15 fits
+// the 16-bit operand): v_mad_u32_u16 selects the half itself.
+__device__ __forceinline__ uint32_t mad_lo16(uint32_t w, uint32_t m, uint32_t c) {
+  uint32_t r;
+  asm("v_mad_u32_u16 %0, %1, %2, %3" : "=v"(r) : "v"(w), "v"(m), "v"(c));
+  return r;
+}
+__device__ __forceinline__ uint32_t mad_hi16(uint32_t w, uint32_t m, uint32_t c) {
+  uint32_t r;
+  asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[1,0,0,0]" : "=v"(r) : "v"(w), "v"(m), "v"(c));
+  return r;
+}
+
+// quantize(x - p) for sample `hi` of the packed input word w, with the work
+// that does not depend on the prediction p done first.  Bit depth <= 12:
+// n * m + negks (mod 2^32) for n = x - p equals (x * m + negks) - p * m, so
+// after p only one signed 24-bit multiply-add and the shift remain
+// (|p| < 2^23, m <= 2^15).  Even d: n < h + K*d exactly when
+// p > x - (h + K*d), and n - 1 takes m off the product.
+template <typename FT, bool kOdd>
+__device__ __forceinline__ int quantize_pred(uint32_t w, bool hi, int p, const Quant& q) {
+  const int x = (int)(hi ? w >> 16 : w & 0xffffu);
+  if constexpr (sizeof(FT) == 2) {
+    const uint32_t cx = hi ? mad_hi16(w, q.m, q.negks) : mad_lo16(w, q.m, q.negks);
+    int base = (int)cx;
+    if (!kOdd) base = p > x - q.offp ? (int)(cx - q.m) : (int)cx;
+    return mad_i24(p, -(int)q.m, base) >> q.s;
+  } else {
+    return quantize<FT, kOdd>(x - p, q);
+  }
 }
 
 template <bool kEncode, typename FT>
@@ -198,8 +252,8 @@ struct BandTypes {
   // equal widths: a lane's output chunk overwrites the input chunk it has
   // just consumed, and one line buffer serves both directions
   static constexpr bool kInPlace = kQi == kQo;
-  static constexpr int kLineQuads = 4 * kBandRows * kRowChunks * kQi;
-  static constexpr int kOutQuads = kInPlace ? 0 : 4 * kBandRows * kRowChunks * kQo;
+  static constexpr int kLineQuads = 4 * kLineRowSlots * kRowChunks * kQi;
+  static constexpr int kOutQuads = kInPlace ? 0 : 4 * kLineRowSlots * kRowChunks * kQo;
 };
 
 // Dynamic LDS of one wave: line buffers, then the four rings.
@@ -288,15 +342,19 @@ __device__ __forceinline__ void band_group(
   }
 
   // ---- LDS
-  v4u* const lin = smem + grp * (kBandRows * kRowChunks * kQi);
+  v4u* const lin = smem + grp * (kLineRowSlots * kRowChunks * kQi);
   v4u* const lout =
-      T::kInPlace ? lin : smem + T::kLineQuads + grp * (kBandRows * kRowChunks * kQo);
-  // (rings one quad apart: the four tiles' slots fall on different banks)
+      T::kInPlace ? lin : smem + T::kLineQuads + grp * (kLineRowSlots * kRowChunks * kQo);
+  // Ring of a tile: quads [0, ring_groups) hold samples 0-3 of each column
+  // group's slot, [ring_groups, 2 ring_groups) samples 4-7, so the lanes'
+  // slot writes (lane i: slot cg - i) fall on consecutive quads.  (Rings one
+  // quad apart: the four tiles' slots fall on different banks.)
   v4u* const ring = smem + T::kLineQuads + T::kOutQuads + grp * (ring_groups * 2 + 1);
-  v4u* const lin_w = lin + ((8 * half) * kRowChunks + c) * kQi;          // + k rows
-  const v4u* const lout_r = lout + ((8 * half) * kRowChunks + c) * kQo;  // + ((k+1)&7) rows
-  const v4u* const lin_row = lin + i * kRowChunks * kQi;
-  v4u* const lout_row = lout + i * kRowChunks * kQo;
+  // row 8 * half + k sits at row slot 9 * half + k
+  v4u* const lin_w = lin + ((9 * half) * kRowChunks + c) * kQi;          // + k rows
+  const v4u* const lout_r = lout + ((9 * half) * kRowChunks + c) * kQo;  // + ((k+1)&7) rows
+  const v4u* const lin_row = lin + (i + half) * kRowChunks * kQi;
+  v4u* const lout_row = lout + (i + half) * kRowChunks * kQo;
 
   const int beta = kEncode ? qz.lo : 0;
   const int lo = kEncode ? qz.lo : 0;
@@ -366,7 +424,7 @@ __device__ __forceinline__ void band_group(
   int bo = 0;
   int L = beta, UL = beta;
   int r0 = beta, r1 = beta, r2 = beta, r3 = beta, r4 = beta, r5 = beta, r6 = beta, r7 = beta;
-  v4u rv0 = ring[0], rv1 = ring[1];  // ring slot of the next group (used by lane 0)
+  v4u rv0 = ring[0], rv1 = ring[ring_groups];  // ring slot of the next group (used by lane 0)
   uint32_t acc = 0;                  // encoder: OR of every active sample
 
   auto group = [&](Vec8<InT>& slot, uint32_t ldb, int k) __attribute__((always_inline)) {
@@ -391,9 +449,9 @@ __device__ __forceinline__ void band_group(
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int A = above_dpp(ringv[j], rr[j]);
-        const int g = L + A - UL;
+        const int g = add_asm(L, A - UL);
         const int p = kMed ? median3(L, A, g) : med3_asm(lo, hi, g);
-        q[j] = quantize<FT, kOdd>((int)cur.get(j) - p, qz);
+        q[j] = quantize_pred<FT, kOdd>(cur.word(j >> 1), (j & 1) != 0, p, qz);
         const int rec = med3_asm(lo, hi, p + __mul24(q[j], qz.d));
         rr[j] = rec;
         L = rec;
@@ -435,7 +493,7 @@ __device__ __forceinline__ void band_group(
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int A = above_dpp(ringv[j], rr[j]);
-        const int g = L + A - UL;
+        const int g = add_asm(L, A - UL);
         const int p = kMed ? median3(L, A, g) : med3_asm(lo, hi, g);
         const int rec = med3_asm(lo, hi, p + qd[j]);
         rr[j] = rec;
@@ -450,9 +508,9 @@ __device__ __forceinline__ void band_group(
     out.put_q(lout_row + pos * kQo);
     // (3) ring slot of this column group
     {
-      v4u* rp = ring + max(cg, 0) * 2;
+      v4u* rp = ring + max(cg, 0);
       rp[0] = v4u{(uint32_t)r0, (uint32_t)r1, (uint32_t)r2, (uint32_t)r3};
-      rp[1] = v4u{(uint32_t)r4, (uint32_t)r5, (uint32_t)r6, (uint32_t)r7};
+      rp[ring_groups] = v4u{(uint32_t)r4, (uint32_t)r5, (uint32_t)r6, (uint32_t)r7};
     }
     // (4) store the lines finished now, rows k + 1 and k + 9 (mod 16)
     {
@@ -488,9 +546,9 @@ __device__ __forceinline__ void band_group(
       // only lane 0 of a tile uses the slot: every lane reads lane 0's, one
       // address per tile (lane i runs i groups behind lane 0)
       const int cg0 = cg + i >= GP ? cg + i - GP : cg + i;
-      const v4u* rq = ring + cg0 * 2;
+      const v4u* rq = ring + cg0;
       rv0 = rq[0];
-      rv1 = rq[1];
+      rv1 = rq[ring_groups];
     }
   };
 
