@@ -54,6 +54,9 @@ def parse_args():
     ap.add_argument("--latency-reps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--two-streams", type=int, default=1,
+                    help="also time encode and decode on two streams at once (0: skip, so "
+                         "a kernel trace holds only launches that ran alone)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (the "
                          "reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -151,7 +154,9 @@ def main():
     ejob = EncodeJob(fargs).allocate()
     ejob.launch()
     streams = ejob.finish()
-    djob = DecodeJob(streams).allocate()
+    # the decoder reads its own copy of the streams, so the two jobs can also
+    # run at once without one writing what the other reads
+    djob = DecodeJob([s.clone() for s in streams]).allocate()
     djob.launch()
     decoded = djob.finish()
     stream_bytes = int(sum(s.numel() for s in streams))
@@ -168,19 +173,55 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    _native.timing_read(reset=True)
-    _native.timing_enable(True)
-    torch.cuda.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
-    _native.timing_enable(False)
-    kernels = _native.timing_read(reset=True)
     ms_per_step = elapsed * 1e3 / args.steps
     luma = WIDTH * HEIGHT * args.frames
     value = luma / (ms_per_step / 1e3) / 1e9
+
+    # ---- the same steps again with a HIP event pair around every kernel
+    # launch (on its launch stream): per-kernel durations for the roofline.
+    # The events cost host time per launch, so this pass is not the headline.
+    _native.timing_read(reset=True)
+    _native.timing_enable(True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    kernel_pass_ms = (time.perf_counter() - t) * 1e3 / args.steps
+    _native.timing_enable(False)
+    kernels = _native.timing_read(reset=True)
+
+    # ---- encode and decode of a step on two streams at once (decode first):
+    # reported beside the headline, which runs them one after the other
+    enc_stream = torch.cuda.Stream(device)
+    dec_stream = torch.cuda.Stream(device)
+
+    def step_two_streams():
+        cur = torch.cuda.current_stream(device)
+        enc_stream.wait_stream(cur)
+        dec_stream.wait_stream(cur)
+        djob.launch(stream=dec_stream)
+        ejob.launch(stream=enc_stream)
+        ejob.finish()
+        djob.finish()
+
+    two_streams = None
+    if args.two_streams:
+        for _ in range(max(1, args.warmup)):
+            step_two_streams()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            step_two_streams()
+        torch.cuda.synchronize()
+        overlap_ms = (time.perf_counter() - t) * 1e3 / args.steps
+        two_streams = {"ms_per_step": round(overlap_ms, 3),
+                       "gpps": round(luma / (overlap_ms / 1e3) / 1e9, 3)}
 
     # encode-only / decode-only throughput of the batched form
     def timed(fn, reps):
@@ -306,6 +347,8 @@ def main():
                                "256x128 tiles; one step = batched encode + batched decode",
                    "frames": args.frames, "width": WIDTH, "height": HEIGHT,
                    "parallelism": "single GPU"},
+        "two_streams": two_streams,
+        "kernel_timing_pass_ms_per_step": round(kernel_pass_ms, 3),
         "encode_gpps_batched": round(luma / enc_s / 1e9, 3),
         "decode_gpps_batched": round(luma / dec_s / 1e9, 3),
         "plain_per_frame": plain,
