@@ -1456,12 +1456,18 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
     hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(0)>), grid, dim3(64), 0, stream, b.frames,
                        b.dshards, b.lists, tiers, b.list_count + 2, b.o0tabs, b.states,
                        (FT*)b.folded, (char*)b.folded, b.sink, b.queues + 4);
-    hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(1)>), grid, dim3(64), 0, stream, b.frames,
-                       b.dshards, b.lists, tiers + b.shard_total, b.list_count + 3, b.o0tabs,
-                       b.states, (FT*)b.folded, (char*)b.folded, b.sink, b.queues + 5);
-    hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(2) - 1>), grid, dim3(64), 0, stream, b.frames,
-                       b.dshards, b.lists, tiers + 2 * b.shard_total, b.list_count + 4, b.o0tabs,
-                       b.states, (FT*)b.folded, (char*)b.folded, b.sink, b.queues + 6);
+    // wider tables: fewer resident waves, and waves beyond them only queue
+    static const int o1 = blocks_per_cu((const void*)k_dec_o0<FT, o0_tier_cap(1)>, 0);
+    static const int o2 = blocks_per_cu((const void*)k_dec_o0<FT, o0_tier_cap(2) - 1>, 0);
+    const int r1 = o1 * device_cus(), r2 = o2 * device_cus();
+    hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(1)>), dim3(min((int)grid.x, r1)), dim3(64), 0,
+                       stream, b.frames, b.dshards, b.lists, tiers + b.shard_total,
+                       b.list_count + 3, b.o0tabs, b.states, (FT*)b.folded, (char*)b.folded,
+                       b.sink, b.queues + 5);
+    hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(2) - 1>), dim3(min((int)grid.x, r2)), dim3(64),
+                       0, stream, b.frames, b.dshards, b.lists, tiers + 2 * b.shard_total,
+                       b.list_count + 4, b.o0tabs, b.states, (FT*)b.folded, (char*)b.folded,
+                       b.sink, b.queues + 6);
   }
   {
     FVH_TIMED("k_dec_rice", stream);
@@ -1473,7 +1479,8 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
   }
   {
     FVH_TIMED("k_dec_other", stream);
-    const int grid = min(b.shard_total, 4096);
+    static const int per_cu = blocks_per_cu((const void*)k_dec_other<FT>, 0);
+    const int grid = min(b.shard_total, per_cu * device_cus());
     hipLaunchKernelGGL((k_dec_other<FT>), dim3(grid), dim3(64), 0, stream, b.frames, b.states,
                        b.dshards, b.lists, b.list_count, b.shard_total, (FT*)b.folded,
                        (char*)b.folded);

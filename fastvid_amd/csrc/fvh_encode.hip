@@ -1639,10 +1639,16 @@ static void encode_pass(const EncodeBuffers& b, int pass, hipStream_t stream) {
     const dim3 grid(min(groups, 32 * device_cus()));
     hipLaunchKernelGGL((k_rans<FT, 16>), grid, dim3(64), 0, stream, b.frames, b.states, b.tiles,
                        b.shards, b.shard_total, folded, b.recs, b.scratch, b.ranks, -1);
-    hipLaunchKernelGGL((k_rans<FT, 64>), grid, dim3(64), 0, stream, b.frames, b.states, b.tiles,
-                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.ranks, 16);
-    hipLaunchKernelGGL((k_rans<FT, 256>), grid, dim3(64), 0, stream, b.frames, b.states, b.tiles,
-                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.ranks, 64);
+    // the wider tables hold fewer waves per compute unit; their instances
+    // stride over every group, so a grid of resident waves loses nothing
+    static const int o64 = blocks_per_cu((const void*)k_rans<FT, 64>, 0);
+    static const int o256 = blocks_per_cu((const void*)k_rans<FT, 256>, 0);
+    hipLaunchKernelGGL((k_rans<FT, 64>), dim3(min(groups, o64 * device_cus())), dim3(64), 0,
+                       stream, b.frames, b.states, b.tiles, b.shards, b.shard_total, folded,
+                       b.recs, b.scratch, b.ranks, 16);
+    hipLaunchKernelGGL((k_rans<FT, 256>), dim3(min(groups, o256 * device_cus())), dim3(64), 0,
+                       stream, b.frames, b.states, b.tiles, b.shards, b.shard_total, folded,
+                       b.recs, b.scratch, b.ranks, 64);
   }
 }
 

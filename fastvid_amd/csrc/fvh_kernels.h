@@ -80,6 +80,10 @@ struct DecodeBuffers {
 
 // Compute units of the current device (cached per device).
 int device_cus();
+// 64-thread blocks of `kernel` one compute unit holds at once: a persistent
+// kernel's grid beyond this times device_cus() only adds waves that start
+// after others have ended, which for an empty work list is pure dispatch.
+int blocks_per_cu(const void* kernel, int dyn_lds);
 
 void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,
                  int32_t* group_first, int tile_total, hipStream_t stream);

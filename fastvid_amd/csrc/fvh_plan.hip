@@ -65,6 +65,16 @@ __global__ __launch_bounds__(256) void k_plan(const FrameDesc* __restrict__ fram
   }
 }
 
+int blocks_per_cu(const void* kernel, int dyn_lds) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, (size_t)dyn_lds) !=
+          hipSuccess ||
+      per_cu <= 0) {
+    per_cu = 16;
+  }
+  return per_cu;
+}
+
 int device_cus() {
   static int cached[64] = {};
   int dev = 0;
