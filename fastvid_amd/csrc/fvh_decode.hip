@@ -817,8 +817,8 @@ __global__ __launch_bounds__(256) void k_o0_parse(const FrameDesc* __restrict__ 
 // of the lane's bytes (see `iteration`).  While every lane of the wave has
 // four symbols left the rounds run unpredicated; a tail loop finishes lanes of
 // uneven length.  Decoded values go to a
-// 32-symbol stage per shard in LDS and leave as one 16-byte store per lane
-// every two rounds.  Errors (decode.cu:536-548): a lane that needs a byte past
+// 128-byte stage per shard in LDS (O0Block) and leave as two 16-byte stores
+// per lane every four rounds (16-bit output; every two for 32-bit).  Errors (decode.cu:536-548): a lane that needs a byte past
 // its end consumes more bytes than it has, so the final check "bytes consumed
 // == lane length and state == 2^23" covers every failure of the reference.
 constexpr int kO0Shards = 16;
@@ -869,12 +869,26 @@ struct O0Stage {
   static constexpr int kWords = (32 * (int)sizeof(FT) / 4) | 1;
 };
 
+// The order-0 decoder's stage holds one 128-byte block of each shard's
+// output (64 symbols of 16 bits, two iterations; 32 of 32 bits, one), so a
+// flush writes whole 128-byte lines: four lanes, two 16-byte stores each.
+// Blocks of 64 bytes left every line half-written between flushes: on 16
+// 4K RGB frames the kernel then wrote 1.03x and fetched 1.2x more bytes
+// (HBM traffic 1.73x its algorithmic bytes, now 1.54x).
+template <typename FT>
+struct O0Block {
+  static constexpr int kPerLane = 32 / (int)sizeof(FT);  // symbols a lane stores
+  static constexpr int kSteps = kPerLane;                 // steps of a block (4 lanes)
+  static constexpr int kIters = kSteps / 8;               // 8-step iterations
+  static constexpr int kWords = 33;                       // 128 bytes + 1 (odd stride)
+};
+
 template <typename FT, int kMax>
 struct O0Lds {
   uint32_t mask[kO0Shards][kO0Buckets];
   uint2 row[kO0Shards][O0Row<kMax>::kPairs];
   uint8_t prefix[kO0Shards][kO0Buckets];
-  uint32_t stage[kO0Shards][O0Stage<FT>::kWords];
+  uint32_t stage[kO0Shards][O0Block<FT>::kWords];
 };
 
 template <typename FT, int kMax>
@@ -993,7 +1007,7 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
       const uint32_t m = mask[b] & ((2u << (x & 31u)) - 1u);
       const uint32_t rank = bcnt_add(m, prefix[b]);
       const uint2 ev = ent[rank];
-      stage[4 * slot_pos + r] = (FT)ev.y;
+      stage[4 * slot_pos + r] = (FT)ev.y;  // slot_pos: step within the block
       const uint32_t c = ev.x & 0x1fffu;
       const int fneg = ((int)(ev.x << 6)) >> 19;  // freq - 4096, sign-extended
       const uint32_t xd = (uint32_t)__mul24(fneg, (int)(x >> kRansTableLog)) + (x - c);
@@ -1008,22 +1022,20 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
       bb <<= sh;
       used += sh;
     };
-    // the 32 symbols of two rounds: lane r stores symbols 8 r .. 8 r + 7
+    // the block's symbols (from step s0): lane r stores kPer of them from
+    // symbol kPer * r, 32 bytes
+    constexpr int kPer = O0Block<FT>::kPerLane;
     auto flush_full = [&](int s0) __attribute__((always_inline)) {
-      FT* const dst = flush_base + (4 * s0 + 8 * r) * flush_step;
-      if constexpr (sizeof(FT) == 2) {
-        *(uint4*)dst = *(const uint4*)(stage + 8 * r);
-      } else {
-        ((uint4*)dst)[0] = ((const uint4*)(stage + 8 * r))[0];
-        ((uint4*)dst)[1] = ((const uint4*)(stage + 8 * r))[1];
-      }
+      FT* const dst = flush_base + (4 * s0 + kPer * r) * flush_step;
+      ((uint4*)dst)[0] = ((const uint4*)(stage + kPer * r))[0];
+      ((uint4*)dst)[1] = ((const uint4*)(stage + kPer * r))[1];
     };
     auto flush = [&](int s0) __attribute__((always_inline)) {
-      const int p0 = 4 * s0 + 8 * r;
-      if (p0 + 8 <= n) {
+      const int p0 = 4 * s0 + kPer * r;
+      if (p0 + kPer <= n) {
         flush_full(s0);
       } else {
-        for (int j = 0; p0 + j < n; ++j) out[p0 + j] = stage[8 * r + j];
+        for (int j = 0; p0 + j < n; ++j) out[p0 + j] = stage[kPer * r + j];
       }
     };
     // Two rounds: the first reads A:B (pos < 16), the second A:B:C (pos < 24).
@@ -1031,13 +1043,14 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
     // moved loads the chunk after D into n0:n1 as the iteration's last memory
     // operation (`refill`), and the next iteration takes it into D at its
     // end, so each load has a whole iteration to land.
-    auto iteration = [&](int s0, bool predicated) __attribute__((always_inline)) {
+    // (h8: the iteration's first step within its block)
+    auto iteration = [&](int s0, int h8, bool predicated) __attribute__((always_inline)) {
       int used_total = 0;
       {
         uint64_t bb = window8_be(a0, a1, b0, pos);
         int used = 0;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) step(bb, used, t, !predicated || s0 + t < nsteps);
+        for (int t = 0; t < 4; ++t) step(bb, used, h8 + t, !predicated || s0 + t < nsteps);
         used_total = used;
       }
       {
@@ -1046,7 +1059,9 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
         uint64_t bb = window8_be(hi_half ? b0 : a0, hi_half ? b1 : a1, hi_half ? c0 : b0, p1 & 15);
         int used = 0;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) step(bb, used, 4 + t, !predicated || s0 + 4 + t < nsteps);
+        for (int t = 0; t < 4; ++t) {
+          step(bb, used, h8 + 4 + t, !predicated || s0 + 4 + t < nsteps);
+        }
         used_total += used;
       }
       consumed += used_total;
@@ -1076,16 +1091,26 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
       __builtin_amdgcn_sched_barrier(0);
     };
 
+    constexpr int kBlock = O0Block<FT>::kSteps;
     int s0 = 0;
-    for (; s0 + 8 <= full_steps; s0 += 8) {
-      iteration(s0, false);
-      // every lane of the wave has all 32 symbols here (full_steps >= s0 + 8):
-      // one unconditional store, so the memory waits stay exact
+    for (; s0 + kBlock <= full_steps; s0 += kBlock) {
+#pragma unroll
+      for (int h = 0; h < O0Block<FT>::kIters; ++h) {
+        iteration(s0 + 8 * h, 8 * h, false);
+        if (h + 1 < O0Block<FT>::kIters) refill();
+      }
+      // every lane of the wave has the whole block here (full_steps >=
+      // s0 + kBlock): unconditional stores, so the memory waits stay exact
       flush_full(s0);
       refill();
     }
-    for (; s0 < wave_steps; s0 += 8) {
-      iteration(s0, true);
+    // (a block past a lane's last step decodes nothing: its steps are not live)
+    for (; s0 < wave_steps; s0 += kBlock) {
+#pragma unroll
+      for (int h = 0; h < O0Block<FT>::kIters; ++h) {
+        iteration(s0 + 8 * h, 8 * h, true);
+        if (h + 1 < O0Block<FT>::kIters) refill();
+      }
       flush(s0);
       refill();
     }
