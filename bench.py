@@ -54,9 +54,9 @@ def parse_args():
     ap.add_argument("--latency-reps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--two-streams", type=int, default=1,
-                    help="also time encode and decode on two streams at once (0: skip, so "
-                         "a kernel trace holds only launches that ran alone)")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="streams the headline's batch jobs are split over (1: every kernel "
+                         "runs alone, as the kernel-trace runs want)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (the "
                          "reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -129,6 +129,7 @@ def main():
     import torch
     from fastvid_amd import _native
     from fastvid_amd.codec import DecodeJob, EncodeJob, frame_args, encode as enc1, decode as dec1
+    from fastvid_amd.codec import side_streams, split_decode_job, split_encode_job
 
     _native.lib()
     device = torch.device("cuda", 0)
@@ -163,19 +164,35 @@ def main():
 
     verify = {"checked": False}
 
-    # ---- timed batched steps: encode batch + decode batch
+    # ---- timed batched steps: encode batch + decode batch.  With --streams
+    # above 1 each batch job runs as that many sub-batches on their own
+    # streams (SplitJob), joined before the job returns.
     def step():
         ejob.launch()
         ejob.finish()
         djob.launch()
         djob.finish()
 
+    head_step = step
+    split_check = None
+    if args.streams > 1:
+        sej = split_encode_job(fargs, args.streams).allocate()
+        sdj = split_decode_job([s.clone() for s in streams], args.streams).allocate()
+
+        def head_step():
+            sej.launch()
+            sej.finish()
+            sdj.launch()
+            sdj.finish()
+
+        head_step()
+        split_check = (sej, sdj)
     for _ in range(args.warmup):
-        step()
+        head_step()
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        head_step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     ms_per_step = elapsed * 1e3 / args.steps
@@ -198,8 +215,9 @@ def main():
 
     # ---- encode and decode of a step on two streams at once (decode first):
     # reported beside the headline, which runs them one after the other
-    enc_stream = torch.cuda.Stream(device)
-    dec_stream = torch.cuda.Stream(device)
+    # (the split jobs' streams: more streams than hardware queues would put
+    # two of them on one queue, one after the other)
+    enc_stream, dec_stream = side_streams(device, 2)
 
     def step_two_streams():
         cur = torch.cuda.current_stream(device)
@@ -211,7 +229,7 @@ def main():
         djob.finish()
 
     two_streams = None
-    if args.two_streams:
+    if args.streams > 1:
         for _ in range(max(1, args.warmup)):
             step_two_streams()
         torch.cuda.synchronize()
@@ -329,6 +347,19 @@ def main():
                          f"{cpu_s:.1f} s, C oracle with {threads} threads"}
         verify = {"checked": True, "frames_checked": min(done, len(host_frames)),
                   "streams_bit_exact": bool(ok_streams), "decode_bit_exact_first": bool(ok_decode)}
+    if split_check is not None:
+        # the split jobs' last outputs against the single jobs' (byte for byte)
+        sej, sdj = split_check
+        split_streams = sej.finish()
+        split_frames = sdj.finish()
+        final_streams = ejob.finish()
+        final_frames = djob.finish()
+        verify["split_streams_equal"] = all(
+            a.numel() == b.numel() and bool(torch.equal(a, b))
+            for a, b in zip(split_streams, final_streams))
+        verify["split_decode_equal"] = all(
+            bool(torch.equal(a, b)) for fa, fb in zip(split_frames, final_frames)
+            for a, b in zip(fa, fb))
 
     line = {
         "metric": "encode+decode Gpixel/s (luma), batched, bit-exact",

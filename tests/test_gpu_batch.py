@@ -45,6 +45,35 @@ def test_mixed_batch_matches_single_frame_oracle(oracle):
             assert np.array_equal(a.cpu().numpy(), b)
 
 
+@pytest.mark.parametrize("parts", (2, 3))
+def test_split_jobs_match_single_jobs_and_oracle(oracle, parts):
+    from fastvid_amd.codec import (EncodeJob, DecodeJob, frame_args, split_decode_job,
+                                   split_encode_job)
+    sources, args = [], []
+    for i, (layout, w, h, bd, q, tile) in enumerate(MIXED):
+        src = noisy_planes(layout, w, h, bd, seed=300 + i)
+        sources.append(src)
+        args.append(frame_args(_cuda(src), layout=layout, bit_depth=bd, quality=q, tile_size=tile))
+    single = EncodeJob(args).allocate().launch().finish()
+    split = split_encode_job(args, parts).allocate()
+    for _ in range(2):  # a second launch of the same job gives the same bytes
+        streams = split.launch().finish()
+        assert len(streams) == len(single)
+        for got, exp in zip(streams, single):
+            assert bytes(got.cpu().numpy().tobytes()) == bytes(exp.cpu().numpy().tobytes())
+    for got, src, (layout, w, h, bd, q, tile) in zip(streams[:2], sources, MIXED):
+        assert bytes(got.cpu().numpy().tobytes()) == oracle.encode(
+            src, layout=layout, bit_depth=bd, quality=q, tile_size=tile)
+    copies = [t.clone() for t in streams]
+    dsplit = split_decode_job(copies, parts).allocate()
+    dsingle = DecodeJob(copies).allocate().launch().finish()
+    for _ in range(2):
+        frames = dsplit.launch().finish()
+        for fa, fb in zip(frames, dsingle):
+            for a, b in zip(fa, fb):
+                assert torch.equal(a, b)
+
+
 def test_public_wrapper_sequence_and_inspect(oracle):
     import fastvid
     frames = [_cuda(noisy_planes("rgb444", 80, 48, 8, seed=s)) for s in range(5)]
