@@ -163,8 +163,6 @@ class GraphDecoder:
                                         device=self.device) for i in range(n)]
             self.results = torch.empty(n * _native.RESULT_BYTES, dtype=torch.uint8,
                                        device=self.device)
-            self._host_results = torch.empty(n * _native.RESULT_BYTES, dtype=torch.uint8,
-                                             pin_memory=True)
             self._ins = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.streams])
             self._outs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.outputs])
             self._frames = [self.planes(i) for i in range(n)]
@@ -218,7 +216,7 @@ class GraphDecoder:
         return tuple(planes)
 
     def finish(self) -> list:
-        status = _read_results(self.results, self._host_results, self.stream)["status"]
+        status = _read_results(self.results, self.stream)["status"]
         bad = np.flatnonzero(status)
         if bad.size:
             first = int(status[bad[0]])
