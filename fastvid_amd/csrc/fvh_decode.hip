@@ -94,19 +94,19 @@ __global__ void k_dheader(FrameDesc* __restrict__ frames, FrameState* __restrict
 }
 
 // Directory entry and shard-header walk of one tile (decode.cu:139-219).
-__global__ __launch_bounds__(256) void k_parse(const FrameDesc* __restrict__ frames,
-                                               FrameState* __restrict__ states,
-                                               const TileDesc* __restrict__ tiles, int tile_total,
-                                               DShard* __restrict__ dshards,
-                                               int32_t* __restrict__ lists,
-                                               int32_t* __restrict__ list_count, int shard_total,
-                                               char* gbase) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= tile_total) return;
+// One thread per tile; the tile's shards are filed into the per-decoder
+// lists with one atomic per wave and list (a wave-wide scan of the lanes'
+// counts), not one per thread: twelve thousand tiles' same-address atomics
+// serialise at the L2.
+__device__ __forceinline__ bool parse_tile(int t, const FrameDesc* __restrict__ frames,
+                                           FrameState* __restrict__ states,
+                                           const TileDesc* __restrict__ tiles,
+                                           DShard* __restrict__ dshards, char* gbase,
+                                           int* counts) {
   const TileDesc td = tiles[t];
   const FrameDesc& f = frames[td.frame];
   FrameState* st = &states[td.frame];
-  if (st->status != 0) return;  // e.g. a header rejected by k_dheader
+  if (st->status != 0) return false;  // e.g. a header rejected by k_dheader
   const uint8_t* enc = as_global(gbase, (const uint8_t*)f.stream);
   const int64_t size = f.stream_capacity;
   const int local = t - f.tile_begin;
@@ -117,23 +117,23 @@ __global__ __launch_bounds__(256) void k_parse(const FrameDesc* __restrict__ fra
       entry[3] != 0 || rd_u32(entry + 4) != (uint32_t)td.x || rd_u32(entry + 8) != (uint32_t)td.y ||
       rd_u32(entry + 12) != (uint32_t)td.width || rd_u32(entry + 16) != (uint32_t)td.height) {
     set_status(st, kStatusDirectory);
-    return;
+    return false;
   }
   const uint64_t off_u64 = rd_u64(entry + 20);
   if (off_u64 > (uint64_t)size) {
     set_status(st, kStatusDirectory);
-    return;
+    return false;
   }
   const int64_t payload_offset = (int64_t)off_u64;
   const int64_t payload_length = rd_u32(entry + 28);
   if (payload_length > size - payload_offset) {
     set_status(st, kStatusDirectory);
-    return;
+    return false;
   }
   if (local == 0) {
     if (payload_offset != directory_end) {
       set_status(st, kStatusDirectory);
-      return;
+      return false;
     }
   } else {
     const uint8_t* prev = entry - kDirectoryEntryBytes;
@@ -142,29 +142,28 @@ __global__ __launch_bounds__(256) void k_parse(const FrameDesc* __restrict__ fra
     if (prev_off > (uint64_t)size || prev_len > (uint64_t)size - prev_off ||
         off_u64 != prev_off + prev_len) {
       set_status(st, kStatusDirectory);
-      return;
+      return false;
     }
   }
   if (local + 1 == T && payload_offset + payload_length != size) {
     set_status(st, kStatusDirectory);
-    return;
+    return false;
   }
   const int64_t samples = (int64_t)td.width * td.height;
   int64_t cursor = payload_offset;
   int64_t decoded = 0;
-  int n_order0 = 0;
   const int64_t tile_end = payload_offset + payload_length;
   for (int ls = 0; ls < td.shard_count; ++ls) {
     if (cursor > tile_end - 3) {
       set_status(st, kStatusShardHeader);
-      return;
+      return false;
     }
     const int mode = enc[cursor];
     const int64_t body_length = rd_u16(enc + cursor + 1);
     const int64_t body_offset = cursor + 3;
     if (mode > 19 || body_length > tile_end - body_offset) {
       set_status(st, kStatusShardHeader);
-      return;
+      return false;
     }
     const int64_t n = samples - decoded < kShardSymbols ? samples - decoded : kShardSymbols;
     DShard d;
@@ -175,24 +174,51 @@ __global__ __launch_bounds__(256) void k_parse(const FrameDesc* __restrict__ fra
     d.body_length = (int32_t)body_length;
     d.count = (int32_t)n;
     dshards[td.first_shard + ls] = d;
-    n_order0 += mode == kModeOrder0;
+    // order-0, other (zero-run, block-pack), Rice
+    const bool rice = mode >= 1 && mode <= kModeRiceBase + kMaxRice;
+    counts[0] += mode == kModeOrder0;
+    counts[2] += rice;
+    counts[1] += mode != kModeOrder0 && !rice;
     decoded += n;
     cursor = body_offset + body_length;
   }
   if (decoded != samples || cursor != tile_end) {
     set_status(st, kStatusShardHeader);
-    return;
+    return false;
   }
-  // file the tile's shards by decoder kernel: order-0, Rice, the rest
-  int n_rice = 0;
-  for (int ls = 0; ls < td.shard_count; ++ls) {
-    const int mode = dshards[td.first_shard + ls].mode;
-    n_rice += mode >= 1 && mode <= kModeRiceBase + kMaxRice;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_parse(const FrameDesc* __restrict__ frames,
+                                               FrameState* __restrict__ states,
+                                               const TileDesc* __restrict__ tiles, int tile_total,
+                                               DShard* __restrict__ dshards,
+                                               int32_t* __restrict__ lists,
+                                               int32_t* __restrict__ list_count, int shard_total,
+                                               char* gbase) {
+  // every lane of a wave reaches the scans below (lanes past the last tile,
+  // or whose tile failed, file nothing)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int counts[3] = {0, 0, 0};
+  bool ok = false;
+  if (t < tile_total) {
+    ok = parse_tile(t, frames, states, tiles, dshards, gbase, counts);
+    if (!ok) counts[0] = counts[1] = counts[2] = 0;
   }
-  const int n_other = td.shard_count - n_order0 - n_rice;
-  int at0 = n_order0 ? atomicAdd(&list_count[0], n_order0) : 0;
-  int at1 = n_other ? atomicAdd(&list_count[1], n_other) : 0;
-  int at5 = n_rice ? atomicAdd(&list_count[kRiceList], n_rice) : 0;
+  if (__builtin_amdgcn_readfirstlane((int)wave_max((uint32_t)(t < tile_total))) == 0) return;
+  // first list slot of this lane's shards in each list
+  auto claim = [&](int n, int32_t* counter) __attribute__((always_inline)) {
+    const int incl = (int)wave_incl_scan((uint32_t)n);
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    int base = 0;
+    if (lane_id() == 0 && total > 0) base = atomicAdd(counter, total);
+    return __builtin_amdgcn_readfirstlane(base) + incl - n;
+  };
+  int at0 = claim(counts[0], &list_count[0]);
+  int at1 = claim(counts[1], &list_count[1]);
+  int at5 = claim(counts[2], &list_count[kRiceList]);
+  if (!ok) return;
+  const TileDesc td = tiles[t];
   for (int ls = 0; ls < td.shard_count; ++ls) {
     const int shard = td.first_shard + ls;
     const int mode = dshards[shard].mode;
