@@ -975,7 +975,7 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   if (largest > cap || largest <= cap_below) return;
   uint8_t* sbase = scratch + (int64_t)(act ? s : 0) * kScratchBytes;
   if (act) {
-    // LDS entry of a rank: {freq << 19, cum | (4096 - freq) << 12, 1/freq
+    // LDS entry of a rank: {freq << 19, cum | (4096 - freq) << 16, 1/freq
     // rounded up to a double}; trunc(double(x) * that) == floor(x / freq) for
     // every renormalised state (x < freq * 2^19 <= 2^31), freq 1 included
     const uint2* tab = (const uint2*)(sbase + kScratchTable);
@@ -988,7 +988,7 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
         rc = __longlong_as_double(__double_as_longlong(rc) + 1);
       }
       const uint64_t rb = (uint64_t)__double_as_longlong(rc);
-      ent[k] = make_uint4(freq << 19, cum | ((4096u - freq) << 12), (uint32_t)rb,
+      ent[k] = make_uint4(freq << 19, cum | ((4096u - freq) << 16), (uint32_t)rb,
                           (uint32_t)(rb >> 32));
     }
   }
@@ -1059,7 +1059,8 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
         const uint32_t xs = x >> sh;
         const double rc = __longlong_as_double((int64_t)(((uint64_t)e[t].w << 32) | e[t].z));
         const uint32_t q = (uint32_t)((double)xs * rc);
-        const uint32_t xn = xs + (e[t].y & 0xfffu) + __umul24(q, e[t].y >> 12);
+        // (16-bit halves: the add and the multiply read them in place)
+        const uint32_t xn = (xs + (e[t].y & 0xffffu)) + __umul24(q, e[t].y >> 16);
         x = live ? xn : x;
       }
       const int total = pend + (int)(wbits >> 3);
