@@ -90,6 +90,16 @@ def test_public_wrapper_sequence_and_inspect(oracle):
         fastvid.encode(3, format="gray", bit_depth=8)
 
 
+def test_split_job_raises_the_failing_part_error():
+    from fastvid_amd.codec import frame_args, split_encode_job
+    good = _cuda(noisy_planes("gray", 64, 32, 8, seed=2))
+    bad = (torch.full((32, 64), 300, dtype=torch.int32).to(torch.uint16).cuda(),)
+    args = [frame_args(p, layout="gray", bit_depth=8, quality=100) for p in (good, good, bad)]
+    job = split_encode_job(args, 2).allocate()
+    with pytest.raises(RuntimeError, match="status 1"):
+        job.launch().finish()
+
+
 def test_batch_error_isolated_to_its_frame_raises():
     from fastvid_amd.codec import encode_batch, frame_args
     good = _cuda(noisy_planes("gray", 64, 32, 8, seed=1))
