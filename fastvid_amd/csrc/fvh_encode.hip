@@ -1507,9 +1507,10 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
 // with no LDS staging: the record is [mode][body u16][sparse table] at
 // kScratchHead, then the lane lengths 0..2 and final states at
 // kScratchLensStates (both written by k_rans), then the four lanes' bytes
-// at the ends of their areas.  Sixteen lanes per record, four records per
-// wave (a wave's descriptor loads serve four records); lane j of a record
-// writes destination 16-byte chunks j, j + 16, ....  A chunk takes, from
+// at the ends of their areas.  kEmitO0Lanes (32) lanes per record, two
+// records per wave (16 lanes took 3 % longer on 4K RGB records of about a
+// kilobyte, 64 lanes 13 %); lane j of a record writes destination 16-byte
+// chunks j, j + kEmitO0Lanes, ....  A chunk takes, from
 // each piece it overlaps (usually one), sixteen bytes loaded as five dwords
 // and byte-aligned, merged under a byte mask; only the chunks at the
 // record's two ends, shared with the neighbouring records, store bytes.
@@ -1524,6 +1525,8 @@ __device__ __forceinline__ uint32_t byte_span_mask(int lo, int hi, int d) {
   return below_h & ~below_l;
 }
 
+constexpr int kEmitO0Lanes = 32;  // lanes per record
+
 template <typename FT>
 __global__ __launch_bounds__(256) void k_emit_o0(const FrameDesc* __restrict__ frames,
                                                  const FrameState* __restrict__ states,
@@ -1532,8 +1535,8 @@ __global__ __launch_bounds__(256) void k_emit_o0(const FrameDesc* __restrict__ f
                                                  const uint8_t* __restrict__ scratch,
                                                  const int64_t* __restrict__ tile_off, char* gbase) {
   using u4a = uint4 __attribute__((aligned(4)));
-  const int j = (int)threadIdx.x & 15;
-  const int s = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
+  const int j = (int)threadIdx.x % kEmitO0Lanes;
+  const int s = (int)(blockIdx.x * (256 / kEmitO0Lanes) + threadIdx.x / kEmitO0Lanes);
   if (s >= shard_total) return;
   const ShardRec rec = recs[s];
   const ShardDesc sd = shards[s];
@@ -1558,7 +1561,7 @@ __global__ __launch_bounds__(256) void k_emit_o0(const FrameDesc* __restrict__ f
   const int sh = (int)((uintptr_t)out & 15);
   uint8_t* const out16 = out - sh;
   const int end = sh + total;
-  for (int c = j; 16 * c < end; c += 16) {
+  for (int c = j; 16 * c < end; c += kEmitO0Lanes) {
     const int o0 = 16 * c - sh;  // record byte of the chunk's first byte
     const int tlo = max(-o0, 0);
     const int thi = min(total - o0, 16);
@@ -1724,7 +1727,8 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
   }
   {
     FVH_TIMED("k_emit_o0", stream);
-    hipLaunchKernelGGL((k_emit_o0<FT>), dim3((b.shard_total + 15) / 16), dim3(256), 0, stream,
+    constexpr int kPerBlock = 256 / kEmitO0Lanes;
+    hipLaunchKernelGGL((k_emit_o0<FT>), dim3((b.shard_total + kPerBlock - 1) / kPerBlock), dim3(256), 0, stream,
                        b.frames, b.states, b.shards, b.shard_total, b.recs, b.scratch, b.tile_off,
                        (char*)b.folded);
   }
