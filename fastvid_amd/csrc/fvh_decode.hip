@@ -1488,7 +1488,10 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
   }
   {
     FVH_TIMED("k_parse", stream);
-    hipLaunchKernelGGL(k_parse, dim3((b.tile_total + 255) / 256), dim3(256), 0, stream, b.frames,
+    // one wave per block: a batch has only thousands of tiles, and blocks of
+    // four waves left most compute units idle (25 -> under 20 us on 16 4K
+    // RGB frames)
+    hipLaunchKernelGGL(k_parse, dim3((b.tile_total + 63) / 64), dim3(64), 0, stream, b.frames,
                        b.states, b.tiles, b.tile_total, b.dshards, b.lists, b.list_count,
                        b.shard_total, (char*)b.folded);
   }
