@@ -56,6 +56,17 @@ class KernelTime(ctypes.Structure):
                 ("total_ms", ctypes.c_double)]
 
 
+class SynthParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int64), ("height", ctypes.c_int64), ("pitch", ctypes.c_int64),
+                ("bit_depth", ctypes.c_int32), ("rect_count", ctypes.c_int32),
+                ("ramp_cx", ctypes.c_float), ("ramp_cy", ctypes.c_float),
+                ("ramp_lo", ctypes.c_float), ("ramp_scale", ctypes.c_float),
+                ("kx", ctypes.c_float * 3), ("ky", ctypes.c_float * 3),
+                ("phase", ctypes.c_float * 3), ("offset", ctypes.c_float),
+                ("noise_sigma", ctypes.c_float), ("noise_seed", ctypes.c_uint64),
+                ("rects_under_noise", ctypes.c_int32), ("rects", (ctypes.c_int32 * 5) * 20)]
+
+
 _LIB = None
 
 
@@ -90,6 +101,10 @@ def lib():
     handle.fvh_timing_enable.argtypes = [i32]
     handle.fvh_timing_read.restype = ctypes.c_int
     handle.fvh_timing_read.argtypes = [p, i32, i32]
+    handle.fvh_set_band_waves.restype = ctypes.c_int
+    handle.fvh_set_band_waves.argtypes = [i32]
+    handle.fvh_synth_plane.restype = ctypes.c_int
+    handle.fvh_synth_plane.argtypes = [p, p, p]
     handle.fvh_build_info.restype = ctypes.c_char_p
     handle.fvh_build_info.argtypes = []
     _LIB = handle
@@ -116,3 +131,12 @@ def timing_read(reset: bool = True) -> dict:
     entries = (KernelTime * 64)()
     n = lib().fvh_timing_read(entries, 64, 1 if reset else 0)
     return {entries[i].name.decode(): (entries[i].launches, entries[i].total_ms) for i in range(n)}
+
+
+def band_waves(set_to: int = 0) -> int:
+    """Persistent waves per compute unit of the band wavefront; ``set_to``
+    > 0 changes it for launches made afterwards.  Returns the value in force."""
+    value = lib().fvh_set_band_waves(int(set_to))
+    if value < 0:
+        raise ValueError("band waves per compute unit must be in 1..64")
+    return value

@@ -91,8 +91,12 @@ class GraphEncoder:
         return planes + self.job.workspace.numel() + self.job.output.numel()
 
     def load(self, frames: Sequence) -> None:
+        """Copies frames into the resident input planes on the current
+        stream, after any replay still reading them (a ``launch()`` not yet
+        finished)."""
         if len(frames) != len(self.inputs):
             raise ValueError("frame count does not match the planned batch")
+        torch.cuda.current_stream(self.device).wait_stream(self.stream)
         for dst, src in zip(self.inputs, frames):
             src = (src,) if isinstance(src, torch.Tensor) else tuple(src)
             if len(src) != len(dst):
@@ -183,8 +187,11 @@ class GraphDecoder:
                 sum(t.numel() * 2 for t in self.outputs))
 
     def load(self, streams: Sequence[torch.Tensor]) -> None:
+        """Copies streams and their sizes into the resident buffers on the
+        current stream, after any replay still reading them."""
         if len(streams) != len(self.streams):
             raise ValueError("stream count does not match the planned batch")
+        torch.cuda.current_stream(self.device).wait_stream(self.stream)
         sizes = []
         for dst, src in zip(self.streams, streams):
             if src.dtype != torch.uint8 or src.dim() != 1:
@@ -233,8 +240,143 @@ class GraphDecoder:
 
 # ------------------------------------------------------------------ autotune
 
-TILE_CANDIDATES = ((64, 64), (128, 64), (128, 128), (256, 64), (256, 128), (256, 256),
-                   (512, 128))
+# Tile geometries the tuner tries (the stream header carries the tile size, so
+# any of them is a valid version-1 stream); 256x128 is the reference default.
+DEFAULT_TILE = (256, 128)
+TILE_CANDIDATES = ((32, 32), (64, 32), (64, 64), (128, 64), (128, 128), (256, 64), (256, 128),
+                   (128, 256), (256, 256))
+# Persistent band-wavefront waves per compute unit (16 lanes per tile, four
+# tiles per wave; the library's default is 16).
+WAVE_CANDIDATES = (4, 8, 12, 16, 24)
+
+
+@dataclass
+class TuneResult:
+    """Outcome of ``autotune``: a tile size per frame format, the band waves
+    per compute unit, and the measured batch times (seconds per replay of
+    the objective) of the chosen and the default configuration."""
+    tiles: dict
+    band_waves: int
+    seconds: float
+    default_seconds: float
+    evaluations: int
+    log: list
+
+    def tile_for(self, spec: "FrameSpec") -> tuple:
+        return self.tiles.get(_group_key(spec), spec.tile_size)
+
+    def apply(self, specs: Sequence["FrameSpec"]) -> list:
+        return [FrameSpec(s.layout, s.width, s.height, s.bit_depth, s.quality,
+                          self.tile_for(s), s.frame_rate) for s in specs]
+
+
+def _group_key(spec: "FrameSpec") -> tuple:
+    return (spec.layout, spec.width, spec.height, spec.bit_depth, spec.quality)
+
+
+def _median_replay(enc: GraphEncoder, dec, reps: int) -> float:
+    """Median of `reps` timings of one replay of the objective (encode, then
+    decode when `dec` is given), HIP events on the replay streams."""
+    times = []
+    for _ in range(reps):
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record(enc.stream)
+        enc.launch()
+        if dec is not None:
+            dec.stream.wait_stream(enc.stream)
+            dec.launch()
+            end.record(dec.stream)
+        else:
+            end.record(enc.stream)
+        end.synchronize()
+        times.append(start.elapsed_time(end) / 1e3)
+    return float(np.median(times))
+
+
+def autotune(specs: Sequence[FrameSpec], frames: Sequence, tile_candidates=TILE_CANDIDATES,
+             wave_candidates=WAVE_CANDIDATES, reps: int = 5, objective: str = "encode+decode",
+             per_format: bool = True, margin: float = 0.01, device=None) -> TuneResult:
+    """Tile geometry per frame format and band waves per compute unit that
+    minimise the graph replay time of the whole batch.
+
+    The objective is one replay of a ``GraphEncoder`` over every frame of the
+    batch (and, for "encode+decode", a ``GraphDecoder`` of its streams).  The
+    search starts from the default (256x128 tiles, the library's wave count)
+    and is greedy: the wave count over ``wave_candidates``, then one tile
+    size for the whole batch, then (``per_format``) each format's tile size
+    with the others held.  A change is kept only if it is at least
+    ``margin`` faster; at the end the chosen and the default configuration
+    are timed again alternately and the default is returned unless the
+    chosen one is faster, so the result is never slower than the default
+    on the batch it was tuned on.  The library's wave count is left at the
+    chosen value."""
+    from . import _native
+    specs = list(specs)
+    if len(frames) != len(specs):
+        raise ValueError("one frame per spec")
+    default_waves = _native.band_waves()
+    groups = sorted({_group_key(s) for s in specs})
+    log = []
+
+    def measure(tiles: dict, waves: int) -> float:
+        _native.band_waves(waves)
+        tuned = [FrameSpec(s.layout, s.width, s.height, s.bit_depth, s.quality,
+                           tiles.get(_group_key(s), DEFAULT_TILE), s.frame_rate) for s in specs]
+        enc = GraphEncoder(tuned, device=device)
+        enc.load(frames)
+        streams = enc.run()
+        dec = None
+        if objective == "encode+decode":
+            dec = GraphDecoder(streams, capacities=[int(t.numel()) for t in streams],
+                               device=device)
+            dec.load(streams)
+            dec.launch()
+            dec.finish()
+        t = _median_replay(enc, dec, reps)
+        log.append({"tiles": {"%s-%dx%d-%db-q%d" % k: list(v) for k, v in tiles.items()},
+                    "band_waves": waves, "seconds": t})
+        del enc, dec
+        return t
+
+    base_tiles = {g: DEFAULT_TILE for g in groups}
+    best_tiles, best_waves = dict(base_tiles), default_waves
+    best = measure(best_tiles, best_waves)
+    for w in wave_candidates:
+        if w == best_waves:
+            continue
+        t = measure(best_tiles, w)
+        if t < best * (1 - margin):
+            best, best_waves = t, w
+    for tile in tile_candidates:
+        if tile == DEFAULT_TILE:
+            continue
+        trial = {g: tuple(tile) for g in groups}
+        t = measure(trial, best_waves)
+        if t < best * (1 - margin):
+            best, best_tiles = t, trial
+    if per_format and len(groups) > 1:
+        for g in groups:
+            for tile in tile_candidates:
+                if tuple(tile) == best_tiles[g]:
+                    continue
+                trial = dict(best_tiles)
+                trial[g] = tuple(tile)
+                t = measure(trial, best_waves)
+                if t < best * (1 - margin):
+                    best, best_tiles = t, trial
+    # the chosen configuration against the default, alternately
+    chosen, default = [], []
+    for _ in range(3):
+        chosen.append(measure(best_tiles, best_waves))
+        default.append(measure(base_tiles, default_waves))
+    t_chosen, t_default = float(np.median(chosen)), float(np.median(default))
+    if t_chosen >= t_default:
+        best_tiles, best_waves, t_chosen = base_tiles, default_waves, t_default
+    _native.band_waves(best_waves)
+    return TuneResult(best_tiles, best_waves, t_chosen, t_default, len(log), log)
+
 
 _TUNED: dict = {}
 
@@ -242,38 +384,25 @@ _TUNED: dict = {}
 def autotune_tile_size(layout: str, width: int, height: int, bit_depth: int, quality: int,
                        candidates=TILE_CANDIDATES, frames=None, reps: int = 3,
                        device=None) -> tuple:
-    """Tile size (from ``candidates``) that encodes this frame format
-    fastest, measured with graph replays on ``frames`` (default: one
-    synthetic frame of the format).  Results are cached per format.  The
-    stream stays a valid version-1 stream at the chosen tile size."""
+    """Tile size for one frame format alone (``autotune`` over a one-format
+    batch of ``frames``, default one synthetic frame, tile size only).
+    Results are cached per format."""
     key = (layout, width, height, bit_depth, quality, tuple(candidates))
     if key in _TUNED:
         return _TUNED[key]
+    dev = torch.device(device) if device is not None else torch.device(
+        "cuda", torch.cuda.current_device())
     if frames is None:
         from . import synthetic
         spec = {"layout": layout, "width": width, "height": height, "bit_depth": bit_depth,
                 "seed": 12345}
         host = synthetic.spec_planes(spec)
-        dev = torch.device(device) if device is not None else torch.device(
-            "cuda", torch.cuda.current_device())
         frames = [tuple(torch.from_numpy(p.astype("int32")).to(torch.uint16).to(dev)
                         for p in host)]
-    best, best_t = None, float("inf")
-    for tile in candidates:
-        spec = FrameSpec(layout, width, height, bit_depth, quality, tuple(tile))
-        enc = GraphEncoder([spec] * len(frames), device=device)
-        enc.load(frames)
-        enc.run()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            enc.launch()
-        enc.stream.synchronize()
-        t = (time.perf_counter() - t0) / reps
-        enc.finish()
-        if t < best_t:
-            best, best_t = tuple(tile), t
-        del enc
-    _TUNED[key] = best
-    return best
-
+    from . import _native
+    waves = _native.band_waves()
+    result = autotune([FrameSpec(layout, width, height, bit_depth, quality)] * len(frames),
+                      frames, tile_candidates=candidates, wave_candidates=(waves,), reps=reps,
+                      objective="encode", per_format=False, device=dev)
+    _TUNED[key] = result.tiles[(layout, width, height, bit_depth, quality)]
+    return _TUNED[key]

@@ -11,8 +11,9 @@ Frames are drawn (seeded) from {640x360, 1280x720, 1920x1080, 2560x1440,
   graph     the same sequence captured once and replayed (GraphEncoder /
             GraphDecoder: resident buffers, device tile queue, headers read
             on the device)
-  tuned     graph encode with each format's tile size picked by
-            autotune_tile_size (streams stay valid at their tile size)
+  tuned     graph encode and decode with the tile size of each format and
+            the band waves per compute unit picked by ``autotune`` on the
+            whole batch's replay time (streams stay valid at their tile size)
 
 Prints one JSON line: luma Gpixel/s of each form, per-frame latency p50/p99
 (a batched frame completes with its batch) and the checks made (every form's
@@ -45,7 +46,7 @@ def main(argv=None) -> int:
 
     import torch
     from fastvid_amd import codec, synthetic
-    from fastvid_amd.batch import FrameSpec, GraphDecoder, GraphEncoder, autotune_tile_size
+    from fastvid_amd.batch import FrameSpec, GraphDecoder, GraphEncoder, autotune
 
     device = torch.device("cuda", 0)
     torch.cuda.set_device(device)
@@ -148,28 +149,38 @@ def main(argv=None) -> int:
     }
 
     if args.autotune:
-        tuned = {}
-        for s in specs:
-            key = (s["layout"], s["width"], s["height"], s["bit_depth"], s["quality"])
-            if key not in tuned:
-                tuned[key] = autotune_tile_size(*key)
-        tspecs = [FrameSpec(s["layout"], s["width"], s["height"], s["bit_depth"], s["quality"],
-                            tuned[(s["layout"], s["width"], s["height"], s["bit_depth"],
-                                   s["quality"])]) for s in specs]
+        from fastvid_amd import _native
+        default_waves = _native.band_waves()
+        tune = autotune(fspecs, frames, reps=args.reps)
+        tspecs = tune.apply(fspecs)
         tenc = GraphEncoder(tspecs)
         tenc.load(frames)
-        tenc.run()
-        t_tuned = timed(tenc.run, args.reps)
         tstreams = tenc.run()
+        t_tuned = timed(tenc.run, args.reps)
+        tstreams = [t.clone() for t in tenc.run()]
         ok = True
         for s, f, st in zip(tspecs, frames, tstreams):
             eager = codec.encode(f, layout=s.layout, bit_depth=s.bit_depth, quality=s.quality,
                                  tile_size=s.tile_size)
             ok &= torch.equal(eager, st)
+        tdec = GraphDecoder(tstreams, capacities=[t.numel() for t in tstreams])
+        tdec.load(tstreams)
+        t_tdec = timed(lambda: (tdec.launch(), tdec.finish()), args.reps)
+        dok = all(torch.equal(a, b) for f, g in zip(tdec.finish(), codec.decode_batch(tstreams))
+                  for a, b in zip(f, g))
         result["encode"]["tuned_graph_gpps"] = round(luma / t_tuned / 1e9, 3)
-        result["encode"]["tuned_graph_frame_ms"] = round(t_tuned * 1e3, 3)
-        result["tuned_tiles"] = {"%s-%dx%d-%db-q%d" % k: list(v) for k, v in tuned.items()}
+        result["decode"]["tuned_graph_gpps"] = round(luma / t_tdec / 1e9, 3)
+        result["autotune"] = {
+            "objective": "graph replay of the whole batch, encode + decode",
+            "tuned_batch_ms": round(tune.seconds * 1e3, 3),
+            "default_batch_ms": round(tune.default_seconds * 1e3, 3),
+            "band_waves_per_cu": tune.band_waves, "default_band_waves_per_cu": default_waves,
+            "evaluations": tune.evaluations,
+            "tiles": {"%s-%dx%d-%db-q%d" % k: list(v) for k, v in tune.tiles.items()},
+        }
         checks["tuned_graph_equals_eager"] = bool(ok)
+        checks["tuned_graph_decode_equals_batched"] = bool(dok)
+        _native.band_waves(default_waves)
 
     print(json.dumps(result))
     return 0

@@ -179,6 +179,10 @@ class EncodeJob:
             n * _native.RESULT_BYTES, dtype=torch.uint8, device=self.device)
         return self
 
+    def device_buffers(self) -> list:
+        """Every device tensor the launch reads or writes."""
+        return [self.workspace, self.output, self.results, *self._keep]
+
     def launch(self, stream=None):
         if self.workspace is None:
             self.allocate()
@@ -310,6 +314,10 @@ class DecodeJob:
         self._frames = [self.planes(i) for i in range(n)]
         return self
 
+    def device_buffers(self) -> list:
+        """Every device tensor the launch reads or writes."""
+        return [self.workspace, self.results, *self.outputs, *self.inputs]
+
     def launch(self, stream=None):
         if self.workspace is None:
             self.allocate()
@@ -389,6 +397,13 @@ class SplitJob:
         for job in self.jobs:
             job.allocate()
         self.streams = side_streams(self.device, len(self.jobs))
+        # Every buffer a part reads or writes was allocated on (or handed in
+        # from) the caller's stream but is used on the part's stream: tell the
+        # caching allocator, so a buffer freed while its part still runs is not
+        # handed to a new tensor before that part's work is done.
+        for job, side in zip(self.jobs, self.streams):
+            for t in job.device_buffers():
+                t.record_stream(side)
         return self
 
     def launch(self, stream=None):
@@ -402,9 +417,18 @@ class SplitJob:
         return self
 
     def finish(self) -> list:
-        out = []
+        """Waits for every part (each part's result read synchronises its
+        stream), then raises the first part's error, if any, or returns the
+        outputs of all parts in order."""
+        out, error = [], None
         for job in self.jobs:
-            out.extend(job.finish())
+            try:
+                out.extend(job.finish())
+            except RuntimeError as err:
+                if error is None:
+                    error = err
+        if error is not None:
+            raise error
         return out
 
 

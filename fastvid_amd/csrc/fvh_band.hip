@@ -32,6 +32,7 @@
 // tile width, tile origin, pitch and residual base a multiple of 8 samples,
 // tile width <= kBandRingMax, planes and pitches small enough for 32-bit
 // buffer offsets.  Other frames use k_wavefront.
+#include <atomic>
 #include <type_traits>
 
 #include "fvh_device.h"
@@ -640,14 +641,22 @@ __global__ __launch_bounds__(64) void k_band(const FrameDesc* __restrict__ frame
   }
 }
 
+static std::atomic<int> g_band_waves_per_cu{kBandWavesPerCuDefault};
+
+int band_waves_per_cu(int set) {
+  if (set > 0) g_band_waves_per_cu.store(set);
+  return g_band_waves_per_cu.load();
+}
+
 template <bool kEncode, typename FT>
 void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* tiles,
                  const int32_t* group_first, int group_total, int ring_groups, FT* folded,
                  int32_t* queues, bool med, bool plain, hipStream_t stream) {
   if (group_total <= 0) return;
   const int lds = band_lds_bytes<kEncode, FT>(ring_groups);
-  // persistent waves: at most what the device holds at once
-  const dim3 grid(min(group_total, 16 * device_cus()));
+  // persistent waves (a tunable count per compute unit): each takes tile
+  // groups of four tiles from the queue until none are left
+  const dim3 grid(min(group_total, band_waves_per_cu(0) * device_cus()));
   if (med) {
     FVH_TIMED(kEncode ? "k_band_enc" : "k_band_dec", stream);
     hipLaunchKernelGGL((k_band<kEncode, true, FT>), grid, dim3(64), lds, stream, frames, states,

@@ -85,6 +85,12 @@ int device_cus();
 // after others have ended, which for an empty work list is pure dispatch.
 int blocks_per_cu(const void* kernel, int dyn_lds);
 
+// Persistent waves per compute unit of the band wavefront (k_band): the
+// launch-geometry knob the autotuner sets beside the tile size.  set > 0
+// changes it for later launches; returns the current value.
+constexpr int kBandWavesPerCuDefault = 16;
+int band_waves_per_cu(int set);
+
 void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,
                  int32_t* group_first, int tile_total, hipStream_t stream);
 void launch_upload_frames(const FrameDesc* host_frames, int frame_count, FrameDesc* device_frames,

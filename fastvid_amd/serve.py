@@ -85,8 +85,10 @@ class Request:
     rid: int
     kind: str                  # "encode" or "decode"
     arrival: float             # seconds since the start of the run
-    planes: tuple = ()         # encode: device uint16 planes
+    planes: tuple = ()         # encode: device uint16 planes (dropped when done)
     kwargs: dict = field(default_factory=dict)
+    seed: int = 0              # encode: seed of the request's frame
+    shape: tuple = ()          # encode: (width, height, layout, bit depth)
     stream: object = None      # decode: StoredStream
     luma: int = 0
     done: float = -1.0
@@ -105,7 +107,7 @@ class Server:
     """Continuous-batching encoder/decoder over ``slots`` HIP streams."""
 
     def __init__(self, device=None, pool_bytes: int = 64 << 30, slots: int = 2,
-                 max_batch: int = 32, keep_outputs: bool = False):
+                 max_batch: int = 32, keep_outputs: bool = False, keep_every: int = 50):
         self.device = torch.device(device) if device is not None else torch.device(
             "cuda", torch.cuda.current_device())
         with torch.cuda.device(self.device):
@@ -115,6 +117,7 @@ class Server:
         self.max_batch = max_batch
         self.queues = {"encode": collections.deque(), "decode": collections.deque()}
         self.keep_outputs = keep_outputs
+        self.keep_every = max(1, int(keep_every))
         self.completed = []
         self.rejected = 0
 
@@ -205,11 +208,12 @@ class Server:
             for i, r in enumerate(batch):
                 res = _native.Result.from_buffer_copy(host, i * _native.RESULT_BYTES)
                 r.status = res.status
-                if res.status == 0 and (self.keep_outputs or r.rid % 50 == 0):
+                if res.status == 0 and (self.keep_outputs or r.rid % self.keep_every == 0):
                     r.output = job.output.narrow(0, int(job.offsets[i]),
                                                  int(res.stream_length)).clone()
                 elif res.status == 0:
                     r.output = int(res.stream_length)
+                r.planes = ()  # the frame is regenerated from its seed if needed again
                 r.done = now
         else:
             outputs, res_t = job
@@ -217,7 +221,7 @@ class Server:
             for i, r in enumerate(batch):
                 res = _native.Result.from_buffer_copy(host, i * _native.RESULT_BYTES)
                 r.status = res.status
-                if self.keep_outputs:
+                if res.status == 0 and (self.keep_outputs or r.rid % self.keep_every == 0):
                     r.output = outputs[i].clone()
                 r.done = now
         self.pool.release(*region)
@@ -270,24 +274,6 @@ class Server:
 
 # ----------------------------------------------------------- the workload
 
-def _frame_bank(device, resolutions, per_format: int = 4, seed: int = 4):
-    """Resident synthetic frames per (shape, format): request seeds pick one."""
-    from . import synthetic
-    formats = (("gray", 8), ("rgb444", 8), ("gray", 10))
-    bank = {}
-    for (w, h) in resolutions:
-        for layout, bd in formats:
-            frames = []
-            for k in range(per_format):
-                spec = {"layout": layout, "width": w, "height": h, "bit_depth": bd,
-                        "seed": seed * 10000 + k}
-                host = synthetic.spec_planes(spec)
-                frames.append(tuple(torch.from_numpy(p.astype(np.int32)).to(torch.uint16)
-                                    .to(device) for p in host))
-            bank[(w, h, layout, bd)] = frames
-    return bank
-
-
 def make_requests(seconds: float, rate: float, seed: int = 7, encode_share: float = 0.7,
                   resolutions=None):
     """Arrival times (Poisson), kinds and frame choices of the open-loop run."""
@@ -308,22 +294,36 @@ def make_requests(seconds: float, rate: float, seed: int = 7, encode_share: floa
     return out
 
 
+def request_frame(req: "Request", device, stream=None) -> tuple:
+    """The frame of an encode request, generated on the device from the
+    request's seed (the same planes every time it is called)."""
+    from . import synthetic
+    w, h, layout, bd = req.shape
+    return synthetic.device_frame(req.seed, w, h, layout, bd, device, stream)
+
+
 def run(seconds: float = 60.0, rate: float = 2000.0, pool_bytes: int = 64 << 30, slots: int = 2,
         max_batch: int = 32, keep_outputs: bool = False, seed: int = 7, device=None,
-        resolutions=None):
-    """Runs the open-loop workload; returns (requests, elapsed seconds, server)."""
+        resolutions=None, keep_every: int = 50):
+    """Runs the open-loop workload; returns (requests, elapsed seconds, server).
+
+    Each encode request's frame is generated on the device from the
+    request's own seed when it arrives (``request_frame``) and dropped when
+    the request completes.  Decode requests replay streams produced earlier:
+    one per shape and format encoded before the run, then every
+    ``keep_every``-th stream the run produces.  With ``keep_outputs`` every
+    output is kept (for ``verify``); otherwise every ``keep_every``-th."""
     from . import synthetic
     resolutions = tuple(resolutions or synthetic.RESOLUTIONS)
     device = torch.device(device) if device is not None else torch.device(
         "cuda", torch.cuda.current_device())
-    bank = _frame_bank(device, resolutions)
     plan = make_requests(seconds, rate, seed, resolutions=resolutions)
-    server = Server(device, pool_bytes, slots, max_batch, keep_outputs)
-    # streams the decode requests replay: one per (shape, format) encoded
-    # before the run, then every 50th stream the run produces
+    server = Server(device, pool_bytes, slots, max_batch, keep_outputs, keep_every)
     store = []
-    for (w, h, layout, bd), frames in bank.items():
-        s = codec.encode(frames[0], layout=layout, bit_depth=bd,
+    formats = (("gray", 8), ("rgb444", 8), ("gray", 10))
+    for k, ((w, h), (layout, bd)) in enumerate((r, f) for r in resolutions for f in formats):
+        planes = synthetic.device_frame(seed * 100000 + k, w, h, layout, bd, device)
+        s = codec.encode(planes, layout=layout, bit_depth=bd,
                          quality=100 if layout == "gray" and bd == 8 else 90).clone()
         store.append(StoredStream(s, codec._read_header(s)))
     torch.cuda.synchronize()
@@ -336,9 +336,10 @@ def run(seconds: float = 60.0, rate: float = 2000.0, pool_bytes: int = 64 << 30,
         while nxt < len(plan) and plan[nxt][0] <= now:
             t, kind, (w, h, layout, bd, q), rseed = plan[nxt]
             if kind == "encode":
-                frames = bank[(w, h, layout, bd)]
-                req = Request(nxt, "encode", t, frames[rseed % len(frames)],
-                              dict(layout=layout, bit_depth=bd, quality=q), luma=w * h)
+                req = Request(nxt, "encode", t, kwargs=dict(layout=layout, bit_depth=bd,
+                                                            quality=q),
+                              luma=w * h, seed=rseed, shape=(w, h, layout, bd))
+                req.planes = request_frame(req, device)
             else:
                 st = store[rseed % len(store)]
                 req = Request(nxt, "decode", t, stream=st,
@@ -349,12 +350,35 @@ def run(seconds: float = 60.0, rate: float = 2000.0, pool_bytes: int = 64 << 30,
         busy = server.poll(now)
         while server.completed:
             r = server.completed.pop()
-            if (r.kind == "encode" and r.status == 0 and r.rid % 50 == 0 and
+            if (r.kind == "encode" and r.status == 0 and r.rid % keep_every == 0 and
                     isinstance(r.output, torch.Tensor) and len(store) < 4096):
                 store.append(StoredStream(r.output, codec._read_header(r.output)))
         if nxt >= len(plan) and not busy:
             break
     return requests, clock(), server
+
+
+def verify(requests, device=None) -> dict:
+    """Checks every kept output against the plain call made alone: an encode
+    request's stream against ``codec.encode`` of its regenerated frame, a
+    decode request's planes against ``codec.decode`` of its stream.  Returns
+    counts; raises AssertionError on the first mismatch."""
+    device = torch.device(device) if device is not None else torch.device(
+        "cuda", torch.cuda.current_device())
+    checked = {"encode": 0, "decode": 0}
+    for r in requests:
+        if not isinstance(r.output, torch.Tensor) or r.status != 0:
+            continue
+        if r.kind == "encode":
+            alone = codec.encode(request_frame(r, device), **r.kwargs)
+            if not torch.equal(alone, r.output):
+                raise AssertionError(f"request {r.rid}: served stream differs from the plain call")
+        else:
+            alone = torch.cat([p.reshape(-1) for p in codec.decode(r.stream.data)])
+            if not torch.equal(alone, r.output):
+                raise AssertionError(f"request {r.rid}: served planes differ from the plain call")
+        checked[r.kind] += 1
+    return checked
 
 
 def summarize(requests, elapsed):
@@ -386,13 +410,21 @@ def main(argv=None) -> int:
     ap.add_argument("--pool-gib", type=float, default=64.0)
     ap.add_argument("--slots", type=int, default=2)
     ap.add_argument("--max-batch", type=int, default=32)
+    ap.add_argument("--keep-every", type=int, default=50,
+                    help="keep (and verify) every n-th output; 1 keeps and checks all")
+    ap.add_argument("--no-verify", action="store_true")
     args = ap.parse_args(argv)
     requests, elapsed, server = run(args.seconds, args.rate, int(args.pool_gib * (1 << 30)),
-                                    args.slots, args.max_batch)
+                                    args.slots, args.max_batch, keep_every=args.keep_every)
     line = {"metric": "served encode/decode, continuous batching", "rate_req_s": args.rate,
             "encode_share": 0.7, "pool_gib": args.pool_gib, "slots": args.slots,
-            "max_batch": args.max_batch, "data": "synthetic"}
+            "max_batch": args.max_batch,
+            "data": "synthetic, each encode request's frame generated on the device from its "
+                    "own seed"}
     line.update(summarize(requests, elapsed))
+    if not args.no_verify:
+        checked = verify(requests)
+        line["verified_equal_to_plain_calls"] = checked
     print(json.dumps(line))
     return 0
 

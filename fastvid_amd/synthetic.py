@@ -111,3 +111,83 @@ def spec_planes(spec: dict) -> tuple:
         return rgb_frame(spec["seed"], w, h)
     sigma = 3.0 if bd == 8 else 12.0
     return (generate(spec["seed"], w, h, bd, sigma),)
+
+
+# ------------------------------------------------- device-generated frames
+
+def _ramp_params(rng: np.random.Generator, width: int, height: int):
+    angle = rng.uniform(0.0, 2.0 * np.pi)
+    cx, cy = np.cos(angle), np.sin(angle)
+    rx = (0.0, cx * (width - 1) / width)
+    ry = (0.0, cy * (height - 1) / height)
+    lo, hi = min(rx) + min(ry), max(rx) + max(ry)
+    scale = 255.0 / (hi - lo) if hi > lo else 0.0
+    return cx, cy, lo, scale
+
+
+def _device_params(rng: np.random.Generator, width: int, height: int, bit_depth: int):
+    """Ramp, sinusoid and rectangle parameters of one frame (host draws)."""
+    from . import _native
+    p = _native.SynthParams()
+    p.width, p.height, p.pitch, p.bit_depth = width, height, width, bit_depth
+    p.ramp_cx, p.ramp_cy, p.ramp_lo, p.ramp_scale = _ramp_params(rng, width, height)
+    for s in range(3):
+        period = rng.uniform(64.0, 512.0)
+        direction = rng.uniform(0.0, 2.0 * np.pi)
+        k = 2.0 * np.pi / period
+        p.kx[s], p.ky[s] = k * np.cos(direction), k * np.sin(direction)
+        p.phase[s] = rng.uniform(0.0, 2.0 * np.pi)
+    p.rect_count = 20
+    for r in range(20):
+        p.rects[r][0] = int(rng.integers(0, width))
+        p.rects[r][1] = int(rng.integers(0, height))
+        p.rects[r][2] = int(rng.integers(16, 401))
+        p.rects[r][3] = int(rng.integers(16, 401))
+        p.rects[r][4] = int(rng.integers(0, 256))
+    return p
+
+
+def device_frame(seed: int, width: int, height: int, layout: str, bit_depth: int, device,
+                 stream=None) -> tuple:
+    """A frame of the serving workload generated on the device from `seed`.
+
+    Same formula as ``generate`` (gray: field + noise, then rectangles; sigma
+    3 at 8 bits, 12 at higher depths, in units of the depth) and
+    ``rgb_frame`` (shared field + rectangles, per channel an N(0, 15) offset
+    and N(0, 2) noise), but the noise comes from a counter-based generator on
+    the device, so the frame is not the host generator's frame of the same
+    seed.  Enqueued on `stream` (default: the current stream)."""
+    import ctypes
+
+    import torch
+
+    from . import _native
+    rng = np.random.default_rng(seed)
+    base = _device_params(rng, width, height, bit_depth)
+    if stream is None:
+        stream = torch.cuda.current_stream(device)
+    handle = ctypes.c_void_p(stream.cuda_stream)
+    planes = []
+    if layout == "rgb444":
+        channels = []
+        for _ in range(3):
+            channels.append((float(rng.normal(0.0, 15.0)), int(rng.integers(0, 1 << 63))))
+        for offset, noise_seed in channels:
+            p = _native.SynthParams.from_buffer_copy(base)
+            p.offset, p.noise_sigma, p.noise_seed, p.rects_under_noise = offset, 2.0, noise_seed, 1
+            planes.append(p)
+    elif layout == "gray":
+        sigma = 3.0 if bit_depth == 8 else 12.0
+        base.noise_sigma = sigma * 255.0 / ((1 << bit_depth) - 1)
+        base.noise_seed = int(rng.integers(0, 1 << 63))
+        planes.append(base)
+    else:
+        raise ValueError("device frames are gray or rgb444")
+    out = []
+    for p in planes:
+        t = torch.empty((height, width), dtype=torch.uint16, device=device)
+        rc = _native.lib().fvh_synth_plane(ctypes.byref(p), ctypes.c_void_p(t.data_ptr()), handle)
+        if rc != 0:
+            raise RuntimeError(f"fvh_synth_plane failed ({rc})")
+        out.append(t)
+    return tuple(out)

@@ -142,6 +142,34 @@ typedef struct fvh_kernel_time {
 int fvh_timing_enable(int32_t enable);
 int fvh_timing_read(fvh_kernel_time* out, int32_t max, int32_t reset);
 
+/* Launch geometry of the predictor / reconstruction wavefront (no
+ * counterpart in the reference, whose predict_tiles_kernel runs one block
+ * per tile, cuda/csrc/encode.cu:780-783): persistent waves per compute unit,
+ * each taking groups of four tiles (16 lanes per tile) from a device queue.
+ * 1..64 sets it for launches enqueued afterwards (graphs keep the value they
+ * were captured with); 0 only reads it.  Returns the value in force
+ * (default 16), or -1 for an argument outside 0..64. */
+int fvh_set_band_waves(int32_t waves_per_cu);
+
+/* Seeded synthetic plane on the device (no counterpart in the reference;
+ * the serving workload generates each request's frame with it).  Writes
+ * height rows of width uint16 samples at `out` (row pitch `pitch` samples):
+ * ramp + three sinusoids + Gaussian noise + rectangles + offset, in 8-bit
+ * units, scaled by (2^bit_depth - 1)/255, rounded and clipped.  Enqueued on
+ * `stream`; FVH_EARG for invalid parameters. */
+typedef struct fvh_synth_params {
+  int64_t width, height, pitch;
+  int32_t bit_depth, rect_count;
+  float ramp_cx, ramp_cy, ramp_lo, ramp_scale;  /* ramp: (cy*y/H - lo)*scale + cx*x/W*scale */
+  float kx[3], ky[3], phase[3];                 /* 20*sin(kx*x + ky*y + phase) each */
+  float offset, noise_sigma;                    /* 8-bit units */
+  uint64_t noise_seed;
+  int32_t rects_under_noise;                    /* 1: rectangles before offset and noise */
+  int32_t rects[20][5];                         /* x, y, w, h, value */
+} fvh_synth_params;
+
+int fvh_synth_plane(const fvh_synth_params* params, void* out, fvh_stream_t stream);
+
 /* Library version string and the offload target it was built for. */
 const char* fvh_build_info(void);
 

@@ -576,7 +576,6 @@ static void emit_task(void* arg, int64_t shard) {
   emit_shard(e->c, shard, e->stream + e->offsets[shard]);
 }
 
-/* fastvid_encode_cuda host orchestration: encode.cu:686-931. */
 int fvo_encode(const uint16_t* const* planes, const int64_t* pitches,
                int64_t plane_count, int64_t width, int64_t height,
                int64_t layout, int64_t bit_depth, int64_t quality,
@@ -584,7 +583,26 @@ int fvo_encode(const uint16_t* const* planes, const int64_t* pitches,
                int64_t tile_height, int threads, uint8_t** out,
                int64_t* out_len, uint32_t* folded_out,
                fvo_analysis* analysis_out, char* msg, size_t msg_len) {
+  return fvo_encode_traced(planes, pitches, plane_count, width, height, layout, bit_depth,
+                           quality, fps_num, fps_den, tile_width, tile_height, threads, out,
+                           out_len, folded_out, analysis_out, NULL, msg, msg_len);
+}
+
+/* fastvid_encode_cuda host orchestration: encode.cu:686-931. */
+int fvo_encode_traced(const uint16_t* const* planes, const int64_t* pitches,
+                      int64_t plane_count, int64_t width, int64_t height,
+                      int64_t layout, int64_t bit_depth, int64_t quality,
+                      int64_t fps_num, int64_t fps_den, int64_t tile_width,
+                      int64_t tile_height, int threads, uint8_t** out,
+                      int64_t* out_len, uint32_t* folded_out,
+                      fvo_analysis* analysis_out, fvo_trace* trace, char* msg,
+                      size_t msg_len) {
   *out = NULL;
+  if (trace) {
+    trace->baseline_payload = trace->refined_payload = -1;
+    trace->branch = FVO_BRANCH_NONE;
+    trace->passes = 0;
+  }
   *out_len = 0;
   /* argument checks: encode.cu:695-716 */
   if (!(layout >= 0 && layout <= 2)) {
@@ -658,14 +676,21 @@ int fvo_encode(const uint16_t* const* planes, const int64_t* pitches,
 
   int rc = 0;
   int32_t step = fvo_quantization_step(layout, bit_depth, quality);
+  int passes = 1;
+  int branch = FVO_BRANCH_NONE;
   int status = run_pass(&c, threads, step, layout != 0);
   const int64_t baseline_payload = payload_bytes(&c);
+  int64_t refined_payload = -1;
   int gray_med = 0, refined = 0;
   /* gray10 MED re-run: encode.cu:818-823 */
   if (!status) {
     gray_med = layout == 0 && bit_depth == 10 && step > 1 &&
         baseline_payload * 10 > total_samples * 3;
-    if (gray_med) status = run_pass(&c, threads, step, 1);
+    if (gray_med) {
+      status = run_pass(&c, threads, step, 1);
+      ++passes;
+      branch = FVO_BRANCH_GRAY10_MED;
+    }
   }
   /* gray16 refinement: encode.cu:824-847 */
   if (!status && layout == 0 && bit_depth == 16 && step > 1) {
@@ -676,15 +701,26 @@ int fvo_encode(const uint16_t* const* planes, const int64_t* pitches,
       const int32_t baseline_step = step;
       step = fvo_refined_gray16_step(quality);
       status = run_pass(&c, threads, step, 0);
+      ++passes;
       if (!status) {
-        const int64_t refined_payload = payload_bytes(&c);
+        refined_payload = payload_bytes(&c);
         refined = direct || refined_payload < total_samples / 10;
+        branch = direct ? (baseline_payload < total_samples / 10 ? FVO_BRANCH_GRAY16_DIRECT_LOW
+                                                                 : FVO_BRANCH_GRAY16_DIRECT_HIGH)
+                        : (refined ? FVO_BRANCH_GRAY16_PROBE_KEPT : FVO_BRANCH_GRAY16_PROBE_REVERTED);
         if (!refined) {
           step = baseline_step;
           status = run_pass(&c, threads, step, 0);
+          ++passes;
         }
       }
     }
+  }
+  if (trace) {
+    trace->baseline_payload = baseline_payload;
+    trace->refined_payload = refined_payload;
+    trace->branch = branch;
+    trace->passes = passes;
   }
   if (status) {
     char text[96];

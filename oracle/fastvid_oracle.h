@@ -79,6 +79,32 @@ int fvo_encode(const uint16_t* const* planes, const int64_t* pitches,
                int64_t* out_len, uint32_t* folded_out,
                fvo_analysis* analysis_out, char* msg, size_t msg_len);
 
+/* Which multi-pass heuristic of encode.cu:812-847 an encode took, with the
+ * payload sums (record bytes over all shards) its thresholds compared. */
+enum {
+  FVO_BRANCH_NONE = 0,              /* one pass */
+  FVO_BRANCH_GRAY10_MED = 1,        /* gray10 lossy re-run with MED (flag 0x40) */
+  FVO_BRANCH_GRAY16_DIRECT_LOW = 2, /* baseline < samples/10: refined (flag 0x80) */
+  FVO_BRANCH_GRAY16_DIRECT_HIGH = 3,/* baseline > samples/2: refined (flag 0x80) */
+  FVO_BRANCH_GRAY16_PROBE_KEPT = 4, /* probe pass below samples/10: refined */
+  FVO_BRANCH_GRAY16_PROBE_REVERTED = 5 /* probe pass not below samples/10: baseline again */
+};
+
+typedef struct {
+  int64_t baseline_payload, refined_payload; /* refined_payload -1 if no probe/refinement */
+  int64_t branch, passes;
+} fvo_trace;
+
+/* fvo_encode plus the heuristic trace (trace may be NULL). */
+int fvo_encode_traced(const uint16_t* const* planes, const int64_t* pitches,
+                      int64_t plane_count, int64_t width, int64_t height,
+                      int64_t layout, int64_t bit_depth, int64_t quality,
+                      int64_t fps_num, int64_t fps_den, int64_t tile_width,
+                      int64_t tile_height, int threads, uint8_t** out,
+                      int64_t* out_len, uint32_t* folded_out,
+                      fvo_analysis* analysis_out, fvo_trace* trace, char* msg,
+                      size_t msg_len);
+
 /* Header fields after validation (fastvid_cuda.cpp:141-183). */
 typedef struct {
   int64_t layout, quality, bit_depth, width, height, tile_width, tile_height;

@@ -29,6 +29,16 @@ class Analysis(ctypes.Structure):
                 ("lane", ctypes.c_int64 * 4)]
 
 
+class Trace(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "baseline_payload", "refined_payload", "branch", "passes")]
+
+
+# branch codes of fvo_trace (which heuristic of encode.cu:812-847 ran)
+BRANCHES = {0: "none", 1: "gray10_med", 2: "gray16_direct_low", 3: "gray16_direct_high",
+            4: "gray16_probe_kept", 5: "gray16_probe_reverted"}
+
+
 class Header(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int64) for name in (
         "layout", "quality", "bit_depth", "width", "height", "tile_width", "tile_height",
@@ -69,6 +79,10 @@ def lib():
         handle.fvo_encode.restype = ctypes.c_int
         handle.fvo_encode.argtypes = [p, p, i64, i64, i64, i64, i64, i64, i64, i64, i64, i64,
                                       ctypes.c_int, p, p, p, p, ctypes.c_char_p, ctypes.c_size_t]
+        handle.fvo_encode_traced.restype = ctypes.c_int
+        handle.fvo_encode_traced.argtypes = [p, p, i64, i64, i64, i64, i64, i64, i64, i64, i64,
+                                             i64, ctypes.c_int, p, p, p, p, p, ctypes.c_char_p,
+                                             ctypes.c_size_t]
         handle.fvo_parse_header.restype = ctypes.c_int
         handle.fvo_parse_header.argtypes = [p, i64, p, ctypes.c_char_p, ctypes.c_size_t]
         handle.fvo_decode.restype = ctypes.c_int
@@ -158,11 +172,14 @@ def _layout_id(layout) -> int:
 
 
 def encode(planes, *, layout=None, bit_depth=10, quality=90, frame_rate=(24, 1),
-           tile_size=(256, 128), threads=1, intermediates=False):
+           tile_size=(256, 128), threads=1, intermediates=False, trace=False):
     """Encode host uint16 planes (numpy arrays) to a version-1 stream (bytes).
 
     With ``intermediates`` the final pass's folded residuals (tile order) and
-    per-shard analysis records are returned as well."""
+    per-shard analysis records are returned as well.  With ``trace`` the
+    last element returned is a dict naming the multi-pass heuristic taken
+    (``branch``, one of BRANCHES' values), the baseline and refined payload
+    sums it compared and the number of passes."""
     planes = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
     if layout is None:
         layout = 0 if len(planes) == 1 else (1 if planes[1].shape[1] != planes[0].shape[1] else 2)
@@ -183,19 +200,25 @@ def encode(planes, *, layout=None, bit_depth=10, quality=90, frame_rate=(24, 1),
         shard_total = ts[-1]["first_shard"] + ts[-1]["shard_count"] if ts else 0
         analysis = (Analysis * max(shard_total, 1))()
         analysis_ptr = ctypes.addressof(analysis)
-    rc = lib().fvo_encode(ctypes.addressof(pointers), ctypes.addressof(pitches), len(planes),
-                          width, height, layout, bit_depth, quality, frame_rate[0], frame_rate[1],
-                          tile_size[0], tile_size[1], threads, ctypes.addressof(out),
-                          ctypes.addressof(out_len), folded_ptr, analysis_ptr, msg, len(msg))
+    tr = Trace()
+    rc = lib().fvo_encode_traced(ctypes.addressof(pointers), ctypes.addressof(pitches),
+                                 len(planes), width, height, layout, bit_depth, quality,
+                                 frame_rate[0], frame_rate[1], tile_size[0], tile_size[1],
+                                 threads, ctypes.addressof(out), ctypes.addressof(out_len),
+                                 folded_ptr, analysis_ptr, ctypes.addressof(tr), msg, len(msg))
     if rc != 0:
         raise OracleError(rc, msg.value.decode())
     data = ctypes.string_at(out.value, out_len.value)
     lib().fvo_free(out)
+    extra = []
     if intermediates:
         records = [{"mode": a.mode, "param": a.param, "body": a.body, "record": a.record,
                     "lane": list(a.lane)} for a in analysis[:shard_total]]
-        return data, folded, records
-    return data
+        extra += [folded, records]
+    if trace:
+        extra.append({"branch": BRANCHES[tr.branch], "baseline_payload": tr.baseline_payload,
+                      "refined_payload": tr.refined_payload, "passes": tr.passes})
+    return (data, *extra) if extra else data
 
 
 def parse_header(stream: bytes) -> dict:

@@ -71,3 +71,16 @@ def noisy_planes(layout, width, height, bit_depth, seed, sigma=None):
             base[y0:y0 + rng.integers(1, 40), x0:x0 + rng.integers(1, 40)] = rng.integers(0, maximum + 1)
         planes.append(np.clip(np.rint(base), 0, maximum).astype(np.uint16))
     return tuple(planes)
+
+
+def lattice_plane(width, height, step, seed=0, density=0.02):
+    """16-bit plane whose samples all lie on multiples of `step`: a smooth
+    8-bit field (generator G, no noise) divided by 8, plus sparse unit
+    spikes, times `step`.  Encoded at the quality whose refined gray16 step
+    equals `step`, the refined pass codes the lattice exactly and its payload
+    drops below the baseline pass's (the gray16 probe is kept)."""
+    from fastvid_amd import synthetic
+    rng = np.random.default_rng(seed)
+    spikes = (rng.random((height, width)) < density).astype(np.int64)
+    base = synthetic.generate(5, width, height, 8, 0.0).astype(np.int64) // 8
+    return np.clip((base + spikes) * step, 0, 65535).astype(np.uint16)

@@ -150,20 +150,25 @@ def test_quantiser_reciprocal_is_exact():
         assert np.array_equal((n * m) >> np.uint64(32), n // np.uint64(d)), d
 
 
-def test_rans_reciprocal_is_exact():
-    """floor(x/f) == mul_hi(x, ceil(2^(31+c)/f)) >> (c-1), c = ceil(log2 f), for
-    x < 2^31 (the encoder state after renormalisation) and 2 <= f <= 4096."""
-    rng = np.random.default_rng(7)
-    x_rand = rng.integers(0, 1 << 31, size=20000, dtype=np.uint64)
-    for f in range(2, 4097):
-        c = int(f - 1).bit_length()
-        m = ((1 << (31 + c)) + f - 1) // f
-        assert m < (1 << 32)
-        k = np.arange(1, (1 << 31) // f, max(1, ((1 << 31) // f) // 64), dtype=np.uint64)
-        x = np.concatenate([x_rand, k * np.uint64(f), k * np.uint64(f) - np.uint64(1),
-                            np.array([(1 << 31) - 1], np.uint64)])
-        q = ((x * np.uint64(m)) >> np.uint64(32)) >> np.uint64(c - 1)
-        assert np.array_equal(q, x // np.uint64(f)), f
+def test_rans_reciprocal_is_exact(tmp_path):
+    """The order-0 encoder's quotient trunc(RN(x * rc_up(f))) equals
+    floor(x / f) for every frequency f in 1..4096 and every renormalised
+    state x < f * 2^19: checked at both ends of every quotient interval
+    (x = k f - 1 and k f, k up to 2^19, 2^32 states), which pins every x
+    because the quotient is monotone in x.  The same check with the
+    reciprocal only rounded to nearest must fail (it does from f = 49)."""
+    import subprocess
+    src = Path(__file__).resolve().parent / "native" / "rans_reciprocal.c"
+    so = tmp_path / "rans_reciprocal.so"
+    subprocess.run(["gcc", "-O2", "-shared", "-fPIC", "-o", str(so), str(src), "-lm"], check=True)
+    lib = ctypes.CDLL(str(so))
+    lib.check_rans_reciprocal.restype = ctypes.c_int64
+    lib.check_rans_reciprocal_nearest.restype = ctypes.c_int64
+    bad, checked = ctypes.c_uint64(), ctypes.c_uint64()
+    first = lib.check_rans_reciprocal(1, 4096, ctypes.byref(bad), ctypes.byref(checked))
+    assert first == 0, (first, bad.value)
+    assert checked.value == 4096 * 2 * (1 << 19)
+    assert lib.check_rans_reciprocal_nearest(1, 4096, ctypes.byref(bad)) != 0
 
 
 def _reference_quantize(r, d):

@@ -1,11 +1,13 @@
 """Serving isolation: a request's result is the same alone and in the mix.
 
-A short open-loop run with continuous batching over two streams; every
-encode request's stream is then compared byte for byte with the plain
-encode of the same frame, and every decode request's planes with the plain
-decode of the same stream.  A second run with a pool too small for two
-batches at once shows admission by footprint still completes every request
-identically.
+Open-loop runs with continuous batching over two streams.  Every encode
+request's stream is compared byte for byte with the plain encode of the same
+frame (regenerated on the device from the request's seed), and every decode
+request's planes with the plain decode of the same stream.  The full-scale
+run uses the served workload itself: all five frame shapes up to 3840x2160,
+three formats, Poisson arrivals at 2000 requests/s and a 64 GiB pool.  A run
+with a pool too small for two batches at once shows that admission by
+footprint still completes every request identically.
 """
 
 import pytest
@@ -17,17 +19,29 @@ SMALL = ((640, 360), (1280, 720))
 
 
 def _check(requests):
-    from fastvid_amd import codec
+    from fastvid_amd.serve import verify
     assert requests and all(r.done >= 0 and r.status == 0 for r in requests)
-    kinds = {r.kind for r in requests}
-    assert kinds == {"encode", "decode"}
-    for r in requests:
-        if r.kind == "encode":
-            alone = codec.encode(r.planes, **r.kwargs)
-            assert torch.equal(alone, r.output), r.rid
-        else:
-            alone = torch.cat([p.reshape(-1) for p in codec.decode(r.stream.data)])
-            assert torch.equal(alone, r.output), r.rid
+    assert {r.kind for r in requests} == {"encode", "decode"}
+    checked = verify(requests)
+    assert checked["encode"] == sum(r.kind == "encode" for r in requests)
+    assert checked["decode"] == sum(r.kind == "decode" for r in requests)
+    return checked
+
+
+def test_served_workload_at_2000_per_second_equals_plain_calls():
+    from fastvid_amd import synthetic
+    from fastvid_amd.serve import run, summarize
+    requests, elapsed, server = run(seconds=1.5, rate=2000.0, pool_bytes=64 << 30, slots=2,
+                                    max_batch=32, keep_outputs=True)
+    shapes = {r.shape[:2] for r in requests if r.kind == "encode"}
+    assert shapes == set(synthetic.RESOLUTIONS)
+    assert {r.shape[2:] for r in requests if r.kind == "encode"} == {
+        ("gray", 8), ("rgb444", 8), ("gray", 10)}
+    assert len(requests) > 2500
+    _check(requests)
+    s = summarize(requests, elapsed)
+    assert s["failed"] == 0 and s["latency_ms_p99"] > 0
+    assert server.pool.used == 0
 
 
 def test_outputs_alone_equal_outputs_in_the_mix():
@@ -46,3 +60,16 @@ def test_small_pool_queues_but_completes():
                               max_batch=8, keep_outputs=True, resolutions=SMALL, seed=11)
     _check(requests)
     assert server.pool.used == 0
+
+
+def test_device_frames_follow_their_seed():
+    from fastvid_amd import synthetic
+    a = synthetic.device_frame(5, 640, 360, "rgb444", 8, "cuda")
+    b = synthetic.device_frame(5, 640, 360, "rgb444", 8, "cuda")
+    c = synthetic.device_frame(6, 640, 360, "rgb444", 8, "cuda")
+    assert all(torch.equal(x, y) for x, y in zip(a, b))
+    assert not torch.equal(a[0], c[0])
+    g = synthetic.device_frame(7, 1280, 720, "gray", 10, "cuda")[0]
+    assert g.dtype == torch.uint16 and g.shape == (720, 1280)
+    v = g.to(torch.int32)
+    assert int(v.max()) <= 1023 and float(v.float().std()) > 10
