@@ -40,6 +40,8 @@ class Result(ctypes.Structure):
 
 
 RESULT_BYTES = ctypes.sizeof(Result)
+# workspace of fvh_encode_launch_static (FVH_STATIC_WORKSPACE_BYTES)
+STATIC_WORKSPACE_BYTES = 256 << 20
 
 STREAM_INFO_FIELDS = (
     "layout", "quality", "bit_depth", "width", "height", "tile_width", "tile_height",
@@ -86,6 +88,8 @@ def lib():
     handle.fvh_encode_plan.argtypes = [p, i32, u64p, u64p, u64p, cs, sz]
     handle.fvh_encode_launch.restype = ctypes.c_int
     handle.fvh_encode_launch.argtypes = [p, i32, p, ctypes.c_uint64, p, ctypes.c_uint64, p, p, cs, sz]
+    handle.fvh_encode_launch_static.restype = ctypes.c_int
+    handle.fvh_encode_launch_static.argtypes = [p, i32, p, ctypes.c_uint64, p, p, cs, sz]
     handle.fvh_decode_header.restype = ctypes.c_int
     handle.fvh_decode_header.argtypes = [p, ctypes.c_int64, p, cs, sz]
     handle.fvh_decode_check_host.restype = ctypes.c_int

@@ -339,12 +339,15 @@ int fvh_encode_plan(const fvh_encode_frame* frames, int32_t frame_count, uint64_
   return FVH_OK;
 }
 
-int fvh_encode_launch(const fvh_encode_frame* frames, int32_t frame_count, void* workspace,
-                      uint64_t workspace_bytes, void* output, uint64_t output_bytes,
-                      fvh_result* results, fvh_stream_t stream, char* msg, size_t msg_len) {
-  EncodePlan plan;
-  int rc = build_encode_plan(frames, frame_count, plan, msg, msg_len);
-  if (rc) return rc;
+}  // extern "C"
+
+namespace fvh {
+namespace {
+
+// Enqueues the encode of a planned batch with workspace `workspace`.
+int encode_launch_planned(EncodePlan& plan, int32_t frame_count, void* workspace,
+                          uint64_t workspace_bytes, void* output, uint64_t output_bytes,
+                          fvh_result* results, fvh_stream_t stream, char* msg, size_t msg_len) {
   if (workspace_bytes < (uint64_t)plan.ws.total || output_bytes < (uint64_t)plan.output_bytes ||
       workspace == nullptr || output == nullptr || results == nullptr) {
     set_msg(msg, msg_len, "workspace or output buffer is too small");
@@ -396,6 +399,45 @@ int fvh_encode_launch(const fvh_encode_frame* frames, int32_t frame_count, void*
     return FVH_EHIP;
   }
   return FVH_OK;
+}
+
+}  // namespace
+}  // namespace fvh
+
+extern "C" {
+
+int fvh_encode_launch(const fvh_encode_frame* frames, int32_t frame_count, void* workspace,
+                      uint64_t workspace_bytes, void* output, uint64_t output_bytes,
+                      fvh_result* results, fvh_stream_t stream, char* msg, size_t msg_len) {
+  EncodePlan plan;
+  const int rc = build_encode_plan(frames, frame_count, plan, msg, msg_len);
+  if (rc) return rc;
+  return encode_launch_planned(plan, frame_count, workspace, workspace_bytes, output,
+                               output_bytes, results, stream, msg, msg_len);
+}
+
+int fvh_encode_launch_static(const fvh_encode_frame* frames, int32_t frame_count, void* output,
+                             uint64_t output_bytes, fvh_result* results, fvh_stream_t stream,
+                             char* msg, size_t msg_len) {
+  EncodePlan plan;
+  const int rc = build_encode_plan(frames, frame_count, plan, msg, msg_len);
+  if (rc) return rc;
+  if ((uint64_t)plan.ws.total > kStaticWorkspaceBytes) {
+    if (msg && msg_len) {
+      std::snprintf(msg, msg_len,
+                    "batch needs %lld workspace bytes, more than the %llu of the static workspace",
+                    (long long)plan.ws.total, (unsigned long long)kStaticWorkspaceBytes);
+    }
+    return FVH_EARG;
+  }
+  void* ws = nullptr;
+  const hipError_t err = static_workspace((hipStream_t)stream, &ws);
+  if (err != hipSuccess) {
+    set_msg(msg, msg_len, hipGetErrorString(err));
+    return FVH_EHIP;
+  }
+  return encode_launch_planned(plan, frame_count, ws, kStaticWorkspaceBytes, output,
+                               output_bytes, results, stream, msg, msg_len);
 }
 
 // fastvid_cuda.cpp:141-183 (messages verbatim).

@@ -85,6 +85,14 @@ int device_cus();
 // after others have ended, which for an empty work list is pure dispatch.
 int blocks_per_cu(const void* kernel, int dyn_lds);
 
+// Bytes of the library-owned encoder workspace of fvh_encode_launch_static
+// (one per device, held until the process ends): one 3840x2160 frame of any
+// layout and depth at the default tiles, or a batch of smaller frames.
+constexpr uint64_t kStaticWorkspaceBytes = 256ull << 20;
+// Device address of that workspace on `stream`'s device (looked up once per
+// device with hipGetSymbolAddress, then cached).
+hipError_t static_workspace(hipStream_t stream, void** out);
+
 // Persistent waves per compute unit of the band wavefront (k_band): the
 // launch-geometry knob the autotuner sets beside the tile size.  set > 0
 // changes it for later launches; returns the current value.

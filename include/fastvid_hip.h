@@ -73,6 +73,23 @@ int fvh_encode_launch(const fvh_encode_frame* frames, int32_t frame_count,
                       uint64_t output_bytes, fvh_result* results,
                       fvh_stream_t stream, char* msg, size_t msg_len);
 
+/* As fvh_encode_launch, with the workspace taken from a library-owned
+ * device array instead of a caller buffer, so the call allocates no device
+ * memory.  No counterpart in the reference (whose encoder allocates its
+ * scratch per call, cuda/csrc/encode.cu:768-773): it is for callers that
+ * cannot allocate -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The array holds
+ * FVH_STATIC_WORKSPACE_BYTES per GPU, from the first call on that GPU until
+ * the process ends; a batch whose fvh_encode_plan workspace is larger is
+ * rejected with FVH_EARG before any launch.  Every call on one GPU shares
+ * that GPU's array, so calls whose work overlaps on one GPU (other streams,
+ * other threads) race: serialise them.  The caller still provides the
+ * output and results buffers (sizes and offsets from fvh_encode_plan). */
+#define FVH_STATIC_WORKSPACE_BYTES (256ull << 20)
+int fvh_encode_launch_static(const fvh_encode_frame* frames, int32_t frame_count, void* output,
+                             uint64_t output_bytes, fvh_result* results, fvh_stream_t stream,
+                             char* msg, size_t msg_len);
+
 /* Header fields of one stream after host-side validation. */
 typedef struct fvh_stream_info {
   int64_t layout, quality, bit_depth, width, height, tile_width, tile_height;
