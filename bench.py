@@ -52,7 +52,7 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--frames", type=int, default=FRAMES)
     ap.add_argument("--latency-reps", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-trials", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--streams", type=int, default=2,
                     help="streams the headline's batch jobs are split over (1: every kernel "
@@ -109,7 +109,7 @@ def algorithmic_bytes(samples, stream_bytes, split):
         "k_band_enc": 2 * s + 2 * s,        # samples in, residuals out
         "k_wavefront_enc": 2 * s + 2 * s,
         "k_analyze": 2 * s,                 # residuals in
-        "k_rans": 2 * s + b,                # residuals in, lane bytes out
+        "k_rans": s + b,                    # rank bytes in (one per sample), lane bytes out
         "k_emit": 2 * ot_s + 2 * ot_b,      # other records: residuals in, records out
         "k_emit_o0": 2 * o0_b,              # order-0 records: scratch pieces in, records out
         "k_dec_o0": o0_b + 2 * o0_s,        # order-0 bodies in, residuals out
@@ -195,6 +195,22 @@ def main():
         head_step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+
+    # ---- latency of the batched form: each batch job is synchronous (its
+    # finish() waits for the result records), so the host clock around it is
+    # the time from launch until every frame of the batch is done
+    enc_job = sej if args.streams > 1 else ejob
+    dec_job = sdj if args.streams > 1 else djob
+    lat_benc, lat_bdec = [], []
+    for _ in range(max(args.steps, 20)):
+        t = time.perf_counter()
+        enc_job.launch()
+        enc_job.finish()
+        t1 = time.perf_counter()
+        dec_job.launch()
+        dec_job.finish()
+        lat_benc.append((t1 - t) * 1e3)
+        lat_bdec.append((time.perf_counter() - t1) * 1e3)
     ms_per_step = elapsed * 1e3 / args.steps
     luma = WIDTH * HEIGHT * args.frames
     value = luma / (ms_per_step / 1e3) / 1e9
@@ -274,6 +290,11 @@ def main():
     def pct(values, q):
         return float(np.percentile(np.array(values), q))
 
+    batched_latency = {"frames_per_batch": args.frames, "batches": len(lat_benc),
+                       "encode_batch_ms_p50": round(pct(lat_benc, 50), 3),
+                       "encode_batch_ms_p99": round(pct(lat_benc, 99), 3),
+                       "decode_batch_ms_p50": round(pct(lat_bdec, 50), 3),
+                       "decode_batch_ms_p99": round(pct(lat_bdec, 99), 3)}
     plain = None
     if lat_enc:
         plain_gpps = WIDTH * HEIGHT / (statistics.median(lat_enc) + statistics.median(lat_dec)) / 1e6
@@ -321,33 +342,54 @@ def main():
                      "unit": "GB/s", "frac": d["hbm_frac"], "traffic": traffic_of("k_band_enc"),
                      "algorithmic_bytes": int(alg["k_band_enc"] / launches_per_step)}
 
-    # ---- CPU baseline: the oracle on host cores, bounded sample.  The
-    # oracle's streams of the sampled frames also check the GPU's, byte for
-    # byte, and its decode of the first one the GPU reconstruction.
+    # ---- CPU baseline: the oracle on host cores, one warm-up then the median
+    # of `cpu_trials` timed trials, at one thread and at every core.  A trial
+    # encodes and decodes `cpu_frames` frames of the workload (the same
+    # frames the GPU coded); the all-core trials also check every GPU stream
+    # against the oracle's byte for byte, and the first decode.
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         threads = len(os.sched_getaffinity(0))
-        done, t = 0, time.perf_counter()
-        ok_streams, ok_decode = True, True
-        while time.perf_counter() - t < args.cpu_seconds or done == 0:
-            i = done % len(host_frames)
-            hf = host_frames[i]
-            s = O.encode(hf, layout="rgb444", bit_depth=8, quality=QUALITY, tile_size=TILE,
-                         threads=threads)
-            planes = O.decode(s, threads=threads)
-            if done < len(host_frames):
-                ok_streams &= bytes(streams[i].cpu().numpy().tobytes()) == s
-                if done == 0:
-                    ok_decode &= all(np.array_equal(a.cpu().numpy(), b)
-                                     for a, b in zip(decoded[i], planes))
-            done += 1
-        cpu_s = time.perf_counter() - t
-        cpu = {"value": round(WIDTH * HEIGHT * done / cpu_s / 1e9, 4), "unit": "Gpixel/s",
+
+        def cpu_trial(nthreads, count):
+            out = []
+            t = time.perf_counter()
+            for i in range(count):
+                hf = host_frames[i % len(host_frames)]
+                s = O.encode(hf, layout="rgb444", bit_depth=8, quality=QUALITY, tile_size=TILE,
+                             threads=nthreads)
+                planes = O.decode(s, threads=nthreads)
+                out.append((s, planes))
+            return time.perf_counter() - t, out
+
+        def cpu_leg(nthreads, count):
+            cpu_trial(nthreads, min(count, 1))  # warm-up
+            times, last = [], None
+            for _ in range(args.cpu_trials):
+                dt, last = cpu_trial(nthreads, count)
+                times.append(dt)
+            return statistics.median(times), times, last
+
+        one_s, one_times, _ = cpu_leg(1, 1)
+        all_count = len(host_frames)
+        all_s, all_times, coded = cpu_leg(threads, all_count)
+        ok_streams = all(bytes(streams[i].cpu().numpy().tobytes()) == coded[i][0]
+                         for i in range(all_count))
+        ok_decode = all(np.array_equal(a.cpu().numpy(), b)
+                        for a, b in zip(decoded[0], coded[0][1]))
+        cpu = {"value": round(WIDTH * HEIGHT * all_count / all_s / 1e9, 4), "unit": "Gpixel/s",
                "cores": threads, "kind": "port",
-               "sample": f"{done} frames of the same workload, encode+decode, "
-                         f"{cpu_s:.1f} s, C oracle with {threads} threads"}
-        verify = {"checked": True, "frames_checked": min(done, len(host_frames)),
+               "sample": f"encode+decode of {all_count} frames of the workload per trial, "
+                         f"median of {args.cpu_trials} trials after a warm-up, "
+                         f"C oracle with {threads} threads",
+               "trial_s": [round(t, 3) for t in all_times],
+               "single_thread": {"value": round(WIDTH * HEIGHT / one_s / 1e9, 5),
+                                 "unit": "Gpixel/s", "cores": 1,
+                                 "sample": f"encode+decode of 1 frame per trial, median of "
+                                           f"{args.cpu_trials} trials after a warm-up",
+                                 "trial_s": [round(t, 3) for t in one_times]}}
+        verify = {"checked": True, "frames_checked": all_count,
                   "streams_bit_exact": bool(ok_streams), "decode_bit_exact_first": bool(ok_decode)}
     if split_check is not None:
         # the split jobs' last outputs against the single jobs' (byte for byte)
@@ -385,6 +427,7 @@ def main():
         "encode_gpps_batched": round(luma / enc_s / 1e9, 3),
         "decode_gpps_batched": round(luma / dec_s / 1e9, 3),
         "plain_per_frame": plain,
+        "batched_latency": batched_latency,
         "stream_bytes_per_frame": stream_bytes // args.frames,
         "verify": verify,
         "roofline": roofline,

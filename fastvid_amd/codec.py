@@ -261,7 +261,15 @@ def _read_header(encoded: torch.Tensor) -> _native.StreamInfo:
 
 
 class DecodeJob:
-    """Validates, allocates and launches the decode of a list of streams."""
+    """Validates, allocates and launches the decode of a list of streams.
+
+    A stream in host memory (the reference's DRAM path,
+    cuda/csrc/fastvid_cuda.cpp:255-356) is checked by the host directory and
+    shard walk here, with the reference's messages, then uploaded by
+    ``launch()`` as an asynchronous copy on the job's stream ahead of the
+    decode kernels: straight from the tensor when it is pinned, otherwise
+    through a pinned staging buffer (torch's caching host allocator, so a
+    staging block is reused only after its copy has completed)."""
 
     def __init__(self, streams: Sequence[torch.Tensor], wavefront: bool = True):
         if not streams:
@@ -269,25 +277,31 @@ class DecodeJob:
         lib = _native.lib()
         self.infos = (_native.StreamInfo * len(streams))()
         self.inputs = []
+        self._uploads = []  # (index, host tensor) of streams uploaded at launch
         device = None
         for i, encoded in enumerate(streams):
             info = _read_header(encoded)
             if encoded.is_cuda:
                 dev_stream = encoded.contiguous()
+                dev = dev_stream.device
             else:
                 host = encoded.contiguous()
                 msg = _native.message_buffer()
                 _native.check(lib.fvh_decode_check_host(
                     ctypes.c_void_p(host.data_ptr()), host.numel(), ctypes.byref(info), msg,
                     len(msg)), msg)
-                dev_stream = host.to(torch.device("cuda", torch.cuda.current_device()))
+                dev_stream = None
+                dev = torch.device("cuda", torch.cuda.current_device())
+                self._uploads.append((i, host))
             if device is None:
-                device = dev_stream.device
-            elif dev_stream.device != device:
+                device = dev
+            elif dev != device:
                 raise RuntimeError("all streams of a batch must use the same CUDA device")
             self.infos[i] = info
             self.inputs.append(dev_stream)
         self.device = device
+        for i, host in self._uploads:
+            self.inputs[i] = torch.empty(host.numel(), dtype=torch.uint8, device=device)
         self.wavefront = bool(wavefront)
         msg = _native.message_buffer()
         ws = ctypes.c_uint64()
@@ -318,6 +332,17 @@ class DecodeJob:
         """Every device tensor the launch reads or writes."""
         return [self.workspace, self.results, *self.outputs, *self.inputs]
 
+    def _upload(self, stream) -> None:
+        """Asynchronous host-to-device copies of the host-resident streams on
+        `stream`."""
+        with torch.cuda.stream(stream):
+            for i, host in self._uploads:
+                if not host.is_pinned():
+                    staging = torch.empty(host.numel(), dtype=torch.uint8, pin_memory=True)
+                    staging.copy_(host)
+                    host = staging
+                self.inputs[i].copy_(host, non_blocking=True)
+
     def launch(self, stream=None):
         if self.workspace is None:
             self.allocate()
@@ -327,6 +352,8 @@ class DecodeJob:
         with torch.cuda.device(self.device):
             stream = stream if stream is not None else torch.cuda.current_stream(self.device)
             self._stream = stream
+            if self._uploads:
+                self._upload(stream)
             _native.check(lib.fvh_decode_launch(
                 ctypes.addressof(self.infos), self._ins, self._outs, n,
                 self.workspace.data_ptr(),
