@@ -21,11 +21,11 @@ _LAYOUT_TO_FORMAT = {0: "gray", 1: "yuv422", 2: "rgb444"}
 
 
 def _is_frame(value) -> bool:
-    return (
-        isinstance(value, (tuple, list))
-        and bool(value)
-        and all(isinstance(plane, torch.Tensor) for plane in value)
-    )
+    """A frame is a non-empty tuple or list of tensors (its planes); anything
+    else passed to ``encode`` is taken as a sequence of frames."""
+    if not isinstance(value, (tuple, list)) or len(value) == 0:
+        return False
+    return all(isinstance(item, torch.Tensor) for item in value)
 
 
 def encode(
@@ -64,27 +64,29 @@ def decode(encoded):
 
 
 def inspect(encoded) -> dict[str, int | str]:
-    """Return self-described frame metadata and exact overhead accounting."""
+    """Return self-described frame metadata and exact overhead accounting.
 
-    if not isinstance(encoded, torch.Tensor) or encoded.dtype != torch.uint8 or encoded.ndim != 1:
+    Reads the 32-byte version-1 header: magic "FVID", version 1, layout byte,
+    depth byte (low six bits: depth - 8), width and height (u32), tile count
+    (u32 at byte 28).  Metadata is the header plus one 32-byte directory
+    entry per tile; a bare stream has no container overhead.  Raises
+    TypeError for anything but a 1-D uint8 tensor and ValueError for a
+    header that is not a supported stream (same contract as the reference,
+    fastvid/__init__.py:66-87)."""
+    if not (isinstance(encoded, torch.Tensor) and encoded.dtype == torch.uint8
+            and encoded.ndim == 1):
         raise TypeError("encoded must be a one-dimensional uint8 tensor")
-    header = bytes(encoded[:32].cpu().numpy())
-    if len(header) != 32 or header[:4] != b"FVID" or header[4] != 1:
+    head = bytes(encoded[:32].cpu().numpy())
+    if len(head) < 32 or not (head.startswith(b"FVID") and head[4] == 1):
         raise ValueError("not a Fastvid supported stream")
-    layout = header[5]
-    if layout not in _LAYOUT_TO_FORMAT:
+    fmt = _LAYOUT_TO_FORMAT.get(head[5])
+    if fmt is None:
         raise ValueError("unknown pixel layout")
-    width, height = struct.unpack_from("<II", header, 8)
-    tile_count = struct.unpack_from("<I", header, 28)[0]
-    return {
-        "width": width,
-        "height": height,
-        "format": _LAYOUT_TO_FORMAT[layout],
-        "bit_depth": (header[7] & 0x3F) + 8,
-        "frame_count": 1,
-        "metadata_bytes": 32 + tile_count * 32,
-        "container_overhead_bytes": 0,
-    }
+    width, height = struct.unpack("<II", head[8:16])
+    (tiles,) = struct.unpack("<I", head[28:32])
+    return {"width": width, "height": height, "format": fmt,
+            "bit_depth": 8 + (head[7] & 0x3F), "frame_count": 1,
+            "metadata_bytes": 32 * (1 + tiles), "container_overhead_bytes": 0}
 
 
 __all__ = ["decode", "encode", "inspect"]

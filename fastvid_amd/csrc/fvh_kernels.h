@@ -96,7 +96,7 @@ hipError_t static_workspace(hipStream_t stream, void** out);
 // Persistent waves per compute unit of the band wavefront (k_band): the
 // launch-geometry knob the autotuner sets beside the tile size.  set > 0
 // changes it for later launches; returns the current value.
-constexpr int kBandWavesPerCuDefault = 16;
+constexpr int kBandWavesPerCuDefault = 8;
 int band_waves_per_cu(int set);
 
 void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,

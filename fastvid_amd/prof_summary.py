@@ -9,6 +9,11 @@
         dispatch of the run, a device-to-device copy of N bytes (so the table
         reports HBM bytes per dispatch in units of a copy's bytes)
 
+    python -m fastvid_amd.prof_summary counters PMC_CSV [PMC_CSV ...] --title T --cmd C
+        per-kernel average per dispatch of every counter in one or more --pmc
+        runs, with the derived VALU and LDS instructions per wave, VALU issue
+        share of wave cycles and LDS bank-conflict share of LDS cycles
+
 The traffic table ends with one machine-readable line per kernel,
 ``traffic: <kernel> <fetch_bytes> <write_bytes>``, which bench.py reads to fill
 its roofline's ``traffic`` field.
@@ -90,6 +95,42 @@ def traffic_table(fetch_csv: str, write_csv: str, copy_bytes: int) -> list[str]:
     return out + ["", "```"] + lines + ["```"]
 
 
+def counter_table(paths) -> list[str]:
+    """{kernel: {counter: mean per dispatch}} over every CSV, as a table."""
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    kernel_of = {}
+    for path in paths:
+        for r in csv.DictReader(open(path)):
+            key = (path, r["Dispatch_Id"])
+            kernel_of[key] = short_name(r["Kernel_Name"])
+            per[key][r["Counter_Name"]] += float(r["Counter_Value"])
+    by_kernel = collections.defaultdict(lambda: collections.defaultdict(list))
+    for key, counters in per.items():
+        for c, v in counters.items():
+            by_kernel[kernel_of[key]][c].append(v)
+    names = sorted({c for k in by_kernel.values() for c in k})
+    out = ["| kernel | " + " | ".join(names) + " |", "|---|" + "---:|" * len(names)]
+    derived = ["", "| kernel | VALU / wave | LDS / wave | VALU busy (ACTIVE_INST_VALU x4 / "
+               "WAVE_CYCLES) | LDS bank-conflict share (BANK_CONFLICT / IDX_ACTIVE) |",
+               "|---|---:|---:|---:|---:|"]
+    for k in sorted(by_kernel):
+        mean = {c: sum(v) / len(v) for c, v in by_kernel[k].items()}
+        out.append("| %s | " % k + " | ".join("%.4g" % mean.get(c, float("nan"))
+                                              for c in names) + " |")
+        waves = mean.get("SQ_WAVES")
+        cells = []
+        for num, den, scale in (("SQ_INSTS_VALU", "SQ_WAVES", 1), ("SQ_INSTS_LDS", "SQ_WAVES", 1),
+                                ("SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES", 4),
+                                ("SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE", 1)):
+            if num in mean and mean.get(den):
+                cells.append("%.3g" % (scale * mean[num] / mean[den]))
+            else:
+                cells.append("-")
+        if waves:
+            derived.append("| %s | " % k + " | ".join(cells) + " |")
+    return out + derived
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="mode", required=True)
@@ -99,15 +140,21 @@ def main(argv=None) -> int:
     t.add_argument("fetch_csv")
     t.add_argument("write_csv")
     t.add_argument("--copy-bytes", type=int, required=True)
-    for p in (s, t):
+    c = sub.add_parser("counters")
+    c.add_argument("csvs", nargs="+")
+    for p in (s, t, c):
         p.add_argument("--title", default="rocprofv3 summary")
         p.add_argument("--cmd", default="")
     a = ap.parse_args(argv)
     out = [f"# {a.title}", ""]
     if a.cmd:
         out += ["Command:", "", "```", a.cmd, "```", ""]
-    out += stats_table(a.csv) if a.mode == "stats" else \
-        traffic_table(a.fetch_csv, a.write_csv, a.copy_bytes)
+    if a.mode == "stats":
+        out += stats_table(a.csv)
+    elif a.mode == "traffic":
+        out += traffic_table(a.fetch_csv, a.write_csv, a.copy_bytes)
+    else:
+        out += counter_table(a.csvs)
     print("\n".join(out))
     return 0
 

@@ -165,7 +165,7 @@ int fvh_timing_read(fvh_kernel_time* out, int32_t max, int32_t reset);
  * each taking groups of four tiles (16 lanes per tile) from a device queue.
  * 1..64 sets it for launches enqueued afterwards (graphs keep the value they
  * were captured with); 0 only reads it.  Returns the value in force
- * (default 16), or -1 for an argument outside 0..64. */
+ * (default 8), or -1 for an argument outside 0..64. */
 int fvh_set_band_waves(int32_t waves_per_cu);
 
 /* Seeded synthetic plane on the device (no counterpart in the reference;
