@@ -917,6 +917,191 @@ struct O0Lds {
   uint32_t stage[kO0Shards][O0Block<FT>::kWords];
 };
 
+// Slot -> symbol of the bucket tables (k_dec_o0): slot -> rank by the
+// boundary mask and prefix count of the slot's 32-slot bucket, rank -> the
+// {cum | (freq - 4096) << 13, value} pair; returns the decoded state
+// freq * (x >> 12) + (x & 4095) - cum before renormalisation, as one signed
+// 24-bit multiply-add (its low 32 bits equal the reference's u32 result).
+template <typename FT, int kMax>
+struct BucketLookup {
+  const uint32_t* mask;
+  const uint8_t* prefix;
+  const uint2* ent;
+  __device__ __forceinline__ uint32_t update(uint32_t x, uint32_t& value) const {
+    const uint32_t b = (x >> 5) & (kO0Buckets - 1);
+    const uint32_t m = mask[b] & ((2u << (x & 31u)) - 1u);
+    const uint32_t rank = bcnt_add(m, prefix[b]);
+    const uint2 ev = ent[rank];
+    value = ev.y;
+    const uint32_t c = ev.x & 0x1fffu;
+    const int fneg = ((int)(ev.x << 6)) >> 19;  // freq - 4096, sign-extended
+    return (uint32_t)__mul24(fneg, (int)(x >> kRansTableLog)) + (x - c);
+  }
+};
+
+// The rANS lanes of one shard, one quad of lanes per shard, lane r running
+// rANS lane r (decode.cu:530-549), with the slot lookup `look`.  Decoded
+// values pass through the shard's 128-byte `stage` row and leave as whole
+// lines; lanes without a shard (item < 0 or a rejected table) decode
+// nothing of use and store to their sink slot.
+template <typename FT, typename Lookup>
+__device__ __forceinline__ void o0_lanes(const Lookup& look, int item, int nsym,
+                                         const O0Tab* tab, int r, int lane,
+                                         const FrameDesc* __restrict__ frames,
+                                         const DShard* __restrict__ dshards,
+                                         const int32_t* __restrict__ lists,
+                                         FrameState* __restrict__ states,
+                                         FT* __restrict__ folded, FT* const stage, char* gbase,
+                                         uint8_t* __restrict__ sink, const void* shared_line) {
+  DShard d = {};
+  if (item >= 0) d = dshards[lists[item]];
+  const FrameDesc& f = frames[d.frame];
+  const uint8_t* body = as_global(gbase, (const uint8_t*)f.stream + d.body_offset);
+  const int length = nsym > 0 ? tab->length[r] : 0;
+  uint32_t x = nsym > 0 ? tab->state[r] : kRansL;
+  // aligned chunks holding the lane's bytes (a lane without bytes reads the
+  // body's first chunk, and never uses it)
+  const uint8_t* first = length > 0 ? body + tab->start[r] : body;
+  const int head = (int)((uintptr_t)first & 15);
+  const uint4* cp = (const uint4*)(first - head);
+  const int last = length > 0 ? (head + length - 1) >> 4 : 0;
+  const Chunk A0 = load_chunk(cp);
+  const Chunk B0 = load_chunk(cp + min(1, last));
+  const Chunk C0 = load_chunk(cp + min(2, last));
+  const Chunk D0 = load_chunk(cp + min(3, last));
+  uint64_t a0 = A0.lo, a1 = A0.hi, b0 = B0.lo, b1 = B0.hi, c0 = C0.lo, c1 = C0.hi;
+  uint64_t d0 = D0.lo, d1 = D0.hi;  // chunk ci + 3, or in flight into n0:n1
+  uint64_t n0 = 0, n1 = 0;
+  bool pend = false;   // n0:n1 holds chunk ci + 3 once its load lands
+  int ci = 0;          // chunk index of A
+  int pos = head;      // next byte within A:B, < 16 when an iteration starts
+  int consumed = 0;    // lane bits consumed
+  const int n = nsym > 0 ? d.count : 0;
+  FT* const out = folded + d.folded_offset;
+  const int nsteps = n > r ? (n - r + 3) >> 2 : 0;
+  // shards without work (past the list's end, rejected tables) do not hold
+  // back the unpredicated rounds; their lanes decode nothing of use and
+  // store to a sink slot
+  const int wave_steps = __builtin_amdgcn_readfirstlane((int)wave_max((uint32_t)nsteps));
+  const int full_steps = min(wave_steps, __builtin_amdgcn_readfirstlane((int)wave_min(
+                                             (uint32_t)(nsym > 0 ? nsteps : 0x7fffffff))));
+  FT* const flush_base = nsym > 0 ? out : (FT*)(sink + lane * kLaneSinkBytes);
+  const int flush_step = nsym > 0 ? 1 : 0;
+  // settle the window loads here: otherwise the wait for them stays at the
+  // top of the loop, where it would also wait for every earlier store
+  __builtin_amdgcn_s_waitcnt(0);
+
+  // one step: slot -> rank -> state update and renormalisation
+  auto step = [&](uint64_t& bb, int& used, int slot_pos, bool live) __attribute__((always_inline)) {
+    uint32_t value;
+    const uint32_t xd = look.update(x, value);
+    stage[4 * slot_pos + r] = (FT)value;  // slot_pos: step within the block
+    // one byte below 2^23, two below 2^15 (never more: xd >= 2^11): the
+    // leading zeros less one, rounded down to a multiple of 8 (none from
+    // 2^31, where a state read from a malformed stream may start)
+    const uint32_t lz = (uint32_t)__builtin_clz(xd);
+    int sh = (int)(__builtin_elementwise_sub_sat(lz, 1u) & 0x18u);
+    if (!live) sh = 0;
+    const uint64_t wide = (((uint64_t)xd << 32) | (uint32_t)(bb >> 32)) << sh;
+    x = live ? (uint32_t)(wide >> 32) : x;
+    bb <<= sh;
+    used += sh;
+  };
+  // the block's symbols (from step s0): lane r stores kPer of them from
+  // symbol kPer * r, 32 bytes
+  constexpr int kPer = O0Block<FT>::kPerLane;
+  auto flush_full = [&](int s0) __attribute__((always_inline)) {
+    FT* const dst = flush_base + (4 * s0 + kPer * r) * flush_step;
+    ((uint4*)dst)[0] = ((const uint4*)(stage + kPer * r))[0];
+    ((uint4*)dst)[1] = ((const uint4*)(stage + kPer * r))[1];
+  };
+  auto flush = [&](int s0) __attribute__((always_inline)) {
+    const int p0 = 4 * s0 + kPer * r;
+    if (p0 + kPer <= n) {
+      flush_full(s0);
+    } else {
+      for (int j = 0; p0 + j < n; ++j) out[p0 + j] = stage[kPer * r + j];
+    }
+  };
+  // Two rounds: the first reads A:B (pos < 16), the second A:B:C (pos < 24).
+  // After them the window moves by one chunk at most (C <- D); a lane that
+  // moved loads the chunk after D into n0:n1 as the iteration's last memory
+  // operation (`refill`), and the next iteration takes it into D at its
+  // end, so each load has a whole iteration to land.
+  // (h8: the iteration's first step within its block)
+  auto iteration = [&](int s0, int h8, bool predicated) __attribute__((always_inline)) {
+    int used_total = 0;
+    {
+      uint64_t bb = window8_be(a0, a1, b0, pos);
+      int used = 0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) step(bb, used, h8 + t, !predicated || s0 + t < nsteps);
+      used_total = used;
+    }
+    {
+      const int p1 = pos + (used_total >> 3);
+      const bool hi_half = p1 >= 16;
+      uint64_t bb = window8_be(hi_half ? b0 : a0, hi_half ? b1 : a1, hi_half ? c0 : b0, p1 & 15);
+      int used = 0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        step(bb, used, h8 + 4 + t, !predicated || s0 + 4 + t < nsteps);
+      }
+      used_total += used;
+    }
+    consumed += used_total;
+    pos += used_total >> 3;
+    const bool adv = pos >= 16;
+    // (kept after the rounds: the wait for last iteration's load goes here)
+    __builtin_amdgcn_sched_barrier(0);
+    d0 = pend ? n0 : d0;
+    d1 = pend ? n1 : d1;
+    a0 = adv ? b0 : a0;
+    a1 = adv ? b1 : a1;
+    b0 = adv ? c0 : b0;
+    b1 = adv ? c1 : b1;
+    c0 = adv ? d0 : c0;
+    c1 = adv ? d1 : c1;
+    pos -= adv ? 16 : 0;
+    ci += adv ? 1 : 0;
+    pend = adv;
+  };
+  // (lanes that did not move read one line every such lane of the wave
+  // shares, so the load stays unconditional and n0:n1 needs no merge)
+  auto refill = [&]() __attribute__((always_inline)) {
+    __builtin_amdgcn_sched_barrier(0);
+    const Chunk Nn = load_chunk(pend ? cp + min(ci + 3, last) : (const uint4*)shared_line);
+    n0 = Nn.lo;
+    n1 = Nn.hi;
+    __builtin_amdgcn_sched_barrier(0);
+  };
+
+  constexpr int kBlock = O0Block<FT>::kSteps;
+  int s0 = 0;
+  for (; s0 + kBlock <= full_steps; s0 += kBlock) {
+#pragma unroll
+    for (int h = 0; h < O0Block<FT>::kIters; ++h) {
+      iteration(s0 + 8 * h, 8 * h, false);
+      if (h + 1 < O0Block<FT>::kIters) refill();
+    }
+    // every lane of the wave has the whole block here (full_steps >=
+    // s0 + kBlock): unconditional stores, so the memory waits stay exact
+    flush_full(s0);
+    refill();
+  }
+  // (a block past a lane's last step decodes nothing: its steps are not live)
+  for (; s0 < wave_steps; s0 += kBlock) {
+#pragma unroll
+    for (int h = 0; h < O0Block<FT>::kIters; ++h) {
+      iteration(s0 + 8 * h, 8 * h, true);
+      if (h + 1 < O0Block<FT>::kIters) refill();
+    }
+    flush(s0);
+    refill();
+  }
+  if (nsym > 0 && (consumed != 8 * length || x != kRansL)) set_status(&states[d.frame], 5);
+}
+
 template <typename FT, int kMax>
 __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ frames,
                                                const DShard* __restrict__ dshards,
@@ -988,162 +1173,97 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
       }
     }
     __builtin_amdgcn_wave_barrier();
-    // ---- lane state
-    DShard d = {};
-    if (item >= 0) d = dshards[lists[item]];
-    const FrameDesc& f = frames[d.frame];
-    const uint8_t* body = as_global(gbase, (const uint8_t*)f.stream + d.body_offset);
-    const int length = nsym > 0 ? tab->length[r] : 0;
-    uint32_t x = nsym > 0 ? tab->state[r] : kRansL;
-    // aligned chunks holding the lane's bytes (a lane without bytes reads the
-    // body's first chunk, and never uses it)
-    const uint8_t* first = length > 0 ? body + tab->start[r] : body;
-    const int head = (int)((uintptr_t)first & 15);
-    const uint4* cp = (const uint4*)(first - head);
-    const int last = length > 0 ? (head + length - 1) >> 4 : 0;
-    const Chunk A0 = load_chunk(cp);
-    const Chunk B0 = load_chunk(cp + min(1, last));
-    const Chunk C0 = load_chunk(cp + min(2, last));
-    const Chunk D0 = load_chunk(cp + min(3, last));
-    uint64_t a0 = A0.lo, a1 = A0.hi, b0 = B0.lo, b1 = B0.hi, c0 = C0.lo, c1 = C0.hi;
-    uint64_t d0 = D0.lo, d1 = D0.hi;  // chunk ci + 3, or in flight into n0:n1
-    uint64_t n0 = 0, n1 = 0;
-    bool pend = false;   // n0:n1 holds chunk ci + 3 once its load lands
-    int ci = 0;          // chunk index of A
-    int pos = head;      // next byte within A:B, < 16 when an iteration starts
-    int consumed = 0;    // lane bits consumed
-    const int n = nsym > 0 ? d.count : 0;
-    FT* const out = folded + d.folded_offset;
-    const int nsteps = n > r ? (n - r + 3) >> 2 : 0;
-    // shards without work (past the list's end, rejected tables) do not hold
-    // back the unpredicated rounds; their lanes decode nothing of use and
-    // store to a sink slot
-    const int wave_steps = __builtin_amdgcn_readfirstlane((int)wave_max((uint32_t)nsteps));
-    const int full_steps = min(wave_steps, __builtin_amdgcn_readfirstlane((int)wave_min(
-                                               (uint32_t)(nsym > 0 ? nsteps : 0x7fffffff))));
-    FT* const flush_base = nsym > 0 ? out : (FT*)(sink + lane * kLaneSinkBytes);
-    const int flush_step = nsym > 0 ? 1 : 0;
-    // settle the window loads here: otherwise the wait for them stays at the
-    // top of the loop, where it would also wait for every earlier store
-    __builtin_amdgcn_s_waitcnt(0);
-
-    // one step: slot -> rank -> state update and renormalisation
-    auto step = [&](uint64_t& bb, int& used, int slot_pos, bool live) __attribute__((always_inline)) {
-      const uint32_t b = (x >> 5) & (kO0Buckets - 1);
-      const uint32_t m = mask[b] & ((2u << (x & 31u)) - 1u);
-      const uint32_t rank = bcnt_add(m, prefix[b]);
-      const uint2 ev = ent[rank];
-      stage[4 * slot_pos + r] = (FT)ev.y;  // slot_pos: step within the block
-      const uint32_t c = ev.x & 0x1fffu;
-      const int fneg = ((int)(ev.x << 6)) >> 19;  // freq - 4096, sign-extended
-      const uint32_t xd = (uint32_t)__mul24(fneg, (int)(x >> kRansTableLog)) + (x - c);
-      // one byte below 2^23, two below 2^15 (never more: xd >= 2^11): the
-      // leading zeros less one, rounded down to a multiple of 8 (none from
-      // 2^31, where a state read from a malformed stream may start)
-      const uint32_t lz = (uint32_t)__builtin_clz(xd);
-      int sh = (int)(__builtin_elementwise_sub_sat(lz, 1u) & 0x18u);
-      if (!live) sh = 0;
-      const uint64_t wide = (((uint64_t)xd << 32) | (uint32_t)(bb >> 32)) << sh;
-      x = live ? (uint32_t)(wide >> 32) : x;
-      bb <<= sh;
-      used += sh;
-    };
-    // the block's symbols (from step s0): lane r stores kPer of them from
-    // symbol kPer * r, 32 bytes
-    constexpr int kPer = O0Block<FT>::kPerLane;
-    auto flush_full = [&](int s0) __attribute__((always_inline)) {
-      FT* const dst = flush_base + (4 * s0 + kPer * r) * flush_step;
-      ((uint4*)dst)[0] = ((const uint4*)(stage + kPer * r))[0];
-      ((uint4*)dst)[1] = ((const uint4*)(stage + kPer * r))[1];
-    };
-    auto flush = [&](int s0) __attribute__((always_inline)) {
-      const int p0 = 4 * s0 + kPer * r;
-      if (p0 + kPer <= n) {
-        flush_full(s0);
-      } else {
-        for (int j = 0; p0 + j < n; ++j) out[p0 + j] = stage[kPer * r + j];
-      }
-    };
-    // Two rounds: the first reads A:B (pos < 16), the second A:B:C (pos < 24).
-    // After them the window moves by one chunk at most (C <- D); a lane that
-    // moved loads the chunk after D into n0:n1 as the iteration's last memory
-    // operation (`refill`), and the next iteration takes it into D at its
-    // end, so each load has a whole iteration to land.
-    // (h8: the iteration's first step within its block)
-    auto iteration = [&](int s0, int h8, bool predicated) __attribute__((always_inline)) {
-      int used_total = 0;
-      {
-        uint64_t bb = window8_be(a0, a1, b0, pos);
-        int used = 0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) step(bb, used, h8 + t, !predicated || s0 + t < nsteps);
-        used_total = used;
-      }
-      {
-        const int p1 = pos + (used_total >> 3);
-        const bool hi_half = p1 >= 16;
-        uint64_t bb = window8_be(hi_half ? b0 : a0, hi_half ? b1 : a1, hi_half ? c0 : b0, p1 & 15);
-        int used = 0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          step(bb, used, h8 + 4 + t, !predicated || s0 + 4 + t < nsteps);
-        }
-        used_total += used;
-      }
-      consumed += used_total;
-      pos += used_total >> 3;
-      const bool adv = pos >= 16;
-      // (kept after the rounds: the wait for last iteration's load goes here)
-      __builtin_amdgcn_sched_barrier(0);
-      d0 = pend ? n0 : d0;
-      d1 = pend ? n1 : d1;
-      a0 = adv ? b0 : a0;
-      a1 = adv ? b1 : a1;
-      b0 = adv ? c0 : b0;
-      b1 = adv ? c1 : b1;
-      c0 = adv ? d0 : c0;
-      c1 = adv ? d1 : c1;
-      pos -= adv ? 16 : 0;
-      ci += adv ? 1 : 0;
-      pend = adv;
-    };
-    // (lanes that did not move read one line every such lane of the wave
-    // shares, so the load stays unconditional and n0:n1 needs no merge)
-    auto refill = [&]() __attribute__((always_inline)) {
-      __builtin_amdgcn_sched_barrier(0);
-      const Chunk Nn = load_chunk(pend ? cp + min(ci + 3, last) : (const uint4*)tabs);
-      n0 = Nn.lo;
-      n1 = Nn.hi;
-      __builtin_amdgcn_sched_barrier(0);
-    };
-
-    constexpr int kBlock = O0Block<FT>::kSteps;
-    int s0 = 0;
-    for (; s0 + kBlock <= full_steps; s0 += kBlock) {
-#pragma unroll
-      for (int h = 0; h < O0Block<FT>::kIters; ++h) {
-        iteration(s0 + 8 * h, 8 * h, false);
-        if (h + 1 < O0Block<FT>::kIters) refill();
-      }
-      // every lane of the wave has the whole block here (full_steps >=
-      // s0 + kBlock): unconditional stores, so the memory waits stay exact
-      flush_full(s0);
-      refill();
-    }
-    // (a block past a lane's last step decodes nothing: its steps are not live)
-    for (; s0 < wave_steps; s0 += kBlock) {
-#pragma unroll
-      for (int h = 0; h < O0Block<FT>::kIters; ++h) {
-        iteration(s0 + 8 * h, 8 * h, true);
-        if (h + 1 < O0Block<FT>::kIters) refill();
-      }
-      flush(s0);
-      refill();
-    }
-    if (nsym > 0 && (consumed != 8 * length || x != kRansL)) set_status(&states[d.frame], 5);
+    // ---- the shard's rANS lanes
+    BucketLookup<FT, kMax> look{mask, prefix, ent};
+    o0_lanes<FT>(look, item, nsym, tab, r, lane, frames, dshards, lists, states, folded, stage,
+                 gbase, sink, tabs);
     int next = 0;
     if (lane == 0) next = atomicAdd(queue, 1);
     grp = (int)gridDim.x + __builtin_amdgcn_readfirstlane(next);
+  }
+}
+
+// Latency form of the order-0 decoder, for batches whose order-0 shards fit
+// the device at once (a frame or a few): four shards per wave, each with a
+// direct 4096-slot table in LDS, entry {cum : 12, freq - 1 : 12, rank : 8},
+// so a step has one dependent LDS read instead of the bucket tables' two:
+// freq * (x >> 12) + (x & 4095) - cum = (freq - 1) * (x >> 12) + ((x >> 12)
+// + (x & 4095) - cum), one unsigned 24-bit multiply-add of operands that
+// depend only on x and the entry.  The symbol's value comes from the rank
+// by a second read off the state chain.  Lanes 16..63 of the wave have no
+// shard.  Shards come from the whole order-0 list (any alphabet of at most
+// 255 symbols), block by block.
+constexpr int kO0DirectShards = 4;
+
+template <typename FT>
+struct O0DirectLds {
+  uint32_t slot[kO0DirectShards][4096];
+  uint32_t value[kO0DirectShards][256];
+  uint32_t cum[kO0DirectShards][257];
+  uint32_t stage[kO0Shards][O0Block<FT>::kWords];
+};
+
+struct DirectLookup {
+  const uint32_t* slot;
+  const uint32_t* value;
+  __device__ __forceinline__ uint32_t update(uint32_t x, uint32_t& v) const {
+    const uint32_t e = slot[x & 4095u];
+    const uint32_t q = x >> kRansTableLog;
+    const uint32_t t = q + (x & 4095u);
+    v = value[e >> 24];
+    return __umul24((e >> 12) & 4095u, q) + (t - (e & 4095u));
+  }
+};
+
+template <typename FT>
+__global__ __launch_bounds__(64) void k_dec_o0_direct(const FrameDesc* __restrict__ frames,
+                                                      const DShard* __restrict__ dshards,
+                                                      const int32_t* __restrict__ lists,
+                                                      const int32_t* __restrict__ o0_count,
+                                                      O0Tab* __restrict__ tabs,
+                                                      FrameState* __restrict__ states,
+                                                      FT* __restrict__ folded, char* gbase,
+                                                      uint8_t* __restrict__ sink) {
+  __shared__ O0DirectLds<FT> lds;
+  const int lane = lane_id();
+  const int g = lane >> 2;
+  const int r = lane & 3;
+  const int gs = g < kO0DirectShards ? g : 0;  // lanes without a shard read shard 0's table
+  const int count = *o0_count;
+  for (int blk = blockIdx.x; blk * kO0DirectShards < count; blk += gridDim.x) {
+    const int at = blk * kO0DirectShards + g;
+    const int item = g < kO0DirectShards && at < count ? at : -1;
+    const O0Tab* tab = tabs + (item >= 0 ? item : 0);
+    const int nsym = item >= 0 ? tab->nsym : 0;
+    uint32_t* const slot = lds.slot[gs];
+    uint32_t* const val = lds.value[gs];
+    uint32_t* const cum = lds.cum[gs];
+    // values and cumulative counts by rank (cum[nsym] = 4096)
+    for (int k = r; k <= nsym; k += 4) {
+      const uint32_t e = tab->ent[k];
+      cum[k] = e & 0x1fffu;
+      if (k < nsym) val[k] = e >> 13;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // slots: lane r fills [1024 r, 1024 r + 1024), symbol by symbol
+    const int lo = 1024 * r, hi = lo + 1024;
+    const int wave_nsym = __builtin_amdgcn_readfirstlane((int)wave_max((uint32_t)nsym));
+    for (int k = 0; k < wave_nsym; ++k) {
+      if (k >= nsym) continue;
+      const int c0 = (int)cum[k], c1 = (int)cum[k + 1];
+      const int a = max(c0, lo), b = min(c1, hi);
+      if (a >= b) continue;
+      const uint32_t e = (uint32_t)c0 | ((uint32_t)(c1 - c0 - 1) << 12) | ((uint32_t)k << 24);
+      int sl = a;
+      for (; sl < b && (sl & 3); ++sl) slot[sl] = e;
+      for (; sl + 4 <= b; sl += 4) *(uint4*)&slot[sl] = make_uint4(e, e, e, e);
+      for (; sl < b; ++sl) slot[sl] = e;
+    }
+    __builtin_amdgcn_wave_barrier();
+    DirectLookup look{slot, val};
+    o0_lanes<FT>(look, item, nsym, tab, r, lane, frames, dshards, lists, states, folded,
+                 (FT*)lds.stage[g], gbase, sink, tabs);
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -1501,7 +1621,18 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
                        b.frames, b.states, b.dshards, b.lists, b.list_count, b.lists + 2 * b.shard_total,
                        b.shard_total, b.o0tabs, (char*)b.folded);
   }
-  {
+  // a batch whose shards all fit the device at once in the latency form
+  // (four shards per wave, direct slot tables) takes it: a frame's decode
+  // time is then one shard's chain of steps, each one LDS read long
+  static const int direct_per_cu = blocks_per_cu((const void*)k_dec_o0_direct<FT>, 0);
+  const int direct_cap = kO0DirectShards * direct_per_cu * device_cus();
+  if (b.shard_total <= direct_cap) {
+    FVH_TIMED("k_dec_o0_direct", stream);
+    const int blocks = (b.shard_total + kO0DirectShards - 1) / kO0DirectShards;
+    hipLaunchKernelGGL((k_dec_o0_direct<FT>), dim3(max(blocks, 1)), dim3(64), 0, stream, b.frames,
+                       b.dshards, b.lists, b.list_count, b.o0tabs, b.states, (FT*)b.folded,
+                       (char*)b.folded, b.sink);
+  } else {
     FVH_TIMED("k_dec_o0", stream);
     // one persistent instance per alphabet tier; each drains its list
     const int groups = (b.shard_total + kO0Shards - 1) / kO0Shards;
