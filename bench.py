@@ -54,9 +54,9 @@ def parse_args():
     ap.add_argument("--latency-reps", type=int, default=3)
     ap.add_argument("--cpu-trials", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--streams", type=int, default=2,
-                    help="streams the headline's batch jobs are split over (1: every kernel "
-                         "runs alone, as the kernel-trace runs want)")
+    ap.add_argument("--streams", type=int, default=1,
+                    help="streams the headline's batch jobs are split over (1, the default: "
+                         "every kernel runs alone, as the kernel-trace runs want)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (the "
                          "reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -244,18 +244,16 @@ def main():
         ejob.finish()
         djob.finish()
 
-    two_streams = None
-    if args.streams > 1:
-        for _ in range(max(1, args.warmup)):
-            step_two_streams()
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(args.steps):
-            step_two_streams()
-        torch.cuda.synchronize()
-        overlap_ms = (time.perf_counter() - t) * 1e3 / args.steps
-        two_streams = {"ms_per_step": round(overlap_ms, 3),
-                       "gpps": round(luma / (overlap_ms / 1e3) / 1e9, 3)}
+    for _ in range(max(1, args.warmup)):
+        step_two_streams()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step_two_streams()
+    torch.cuda.synchronize()
+    overlap_ms = (time.perf_counter() - t) * 1e3 / args.steps
+    two_streams = {"ms_per_step": round(overlap_ms, 3),
+                   "gpps": round(luma / (overlap_ms / 1e3) / 1e9, 3)}
 
     # encode-only / decode-only throughput of the batched form
     def timed(fn, reps):

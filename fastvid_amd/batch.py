@@ -297,7 +297,8 @@ def _median_replay(enc: GraphEncoder, dec, reps: int) -> float:
 
 def autotune(specs: Sequence[FrameSpec], frames: Sequence, tile_candidates=TILE_CANDIDATES,
              wave_candidates=WAVE_CANDIDATES, reps: int = 5, objective: str = "encode+decode",
-             per_format: bool = True, margin: float = 0.01, device=None) -> TuneResult:
+             per_format: bool = True, margin: float = 0.01, device=None,
+             progress=None) -> TuneResult:
     """Tile geometry per frame format and band waves per compute unit that
     minimise the graph replay time of the whole batch.
 
@@ -311,7 +312,8 @@ def autotune(specs: Sequence[FrameSpec], frames: Sequence, tile_candidates=TILE_
     are timed again alternately and the default is returned unless the
     chosen one is faster, so the result is never slower than the default
     on the batch it was tuned on.  The library's wave count is left at the
-    chosen value."""
+    chosen value.  ``progress``, if given, is called with each measurement's
+    log entry."""
     from . import _native
     specs = list(specs)
     if len(frames) != len(specs):
@@ -337,6 +339,8 @@ def autotune(specs: Sequence[FrameSpec], frames: Sequence, tile_candidates=TILE_
         t = _median_replay(enc, dec, reps)
         log.append({"tiles": {"%s-%dx%d-%db-q%d" % k: list(v) for k, v in tiles.items()},
                     "band_waves": waves, "seconds": t})
+        if progress is not None:
+            progress(log[-1])
         del enc, dec
         return t
 

@@ -151,7 +151,9 @@ def main(argv=None) -> int:
     if args.autotune:
         from fastvid_amd import _native
         default_waves = _native.band_waves()
-        tune = autotune(fspecs, frames, reps=args.reps)
+        tune = autotune(fspecs, frames, reps=args.reps,
+                        progress=lambda e: print("autotune: %d waves, %.3f ms" % (
+                            e["band_waves"], e["seconds"] * 1e3), file=sys.stderr, flush=True))
         tspecs = tune.apply(fspecs)
         tenc = GraphEncoder(tspecs)
         tenc.load(frames)

@@ -95,8 +95,15 @@ def traffic_table(fetch_csv: str, write_csv: str, copy_bytes: int) -> list[str]:
     return out + ["", "```"] + lines + ["```"]
 
 
-def counter_table(paths) -> list[str]:
-    """{kernel: {counter: mean per dispatch}} over every CSV, as a table."""
+def counter_table(paths, cus: int = 256, xcds: int = 8) -> list[str]:
+    """Per-kernel mean per dispatch of every counter over the CSVs (passes of
+    the same workload, matched by kernel name), then derived shares:
+    kernel cycles = GRBM_GUI_ACTIVE / xcds (the counter sums the XCDs);
+    VALU busy = SQ_INSTS_VALU / (cus * cycles) (a wave64 VALU instruction
+    holds one of a CU's four SIMDs for four cycles); LDS busy =
+    SQ_LDS_IDX_ACTIVE / (cus * cycles); bank-conflict share of LDS cycles;
+    share of wave cycles spent waiting to issue (SQ_WAIT_INST_ANY /
+    SQ_WAVE_CYCLES); L2 miss share of L2 requests."""
     per = collections.defaultdict(lambda: collections.defaultdict(float))
     kernel_of = {}
     for path in paths:
@@ -110,24 +117,31 @@ def counter_table(paths) -> list[str]:
             by_kernel[kernel_of[key]][c].append(v)
     names = sorted({c for k in by_kernel.values() for c in k})
     out = ["| kernel | " + " | ".join(names) + " |", "|---|" + "---:|" * len(names)]
-    derived = ["", "| kernel | VALU / wave | LDS / wave | VALU busy (ACTIVE_INST_VALU x4 / "
-               "WAVE_CYCLES) | LDS bank-conflict share (BANK_CONFLICT / IDX_ACTIVE) |",
-               "|---|---:|---:|---:|---:|"]
+    derived = ["", f"Derived ({cus} compute units, {xcds} XCDs):", "",
+               "| kernel | cycles | VALU instr / wave | VALU busy | LDS busy | LDS bank-conflict "
+               "share | wave wait share | L2 miss share |",
+               "|---|---:|---:|---:|---:|---:|---:|---:|"]
+
+    def ratio(mean, num, den, scale=1.0):
+        if num in mean and mean.get(den):
+            return "%.3g" % (scale * mean[num] / mean[den])
+        return "-"
+
     for k in sorted(by_kernel):
         mean = {c: sum(v) / len(v) for c, v in by_kernel[k].items()}
         out.append("| %s | " % k + " | ".join("%.4g" % mean.get(c, float("nan"))
                                               for c in names) + " |")
-        waves = mean.get("SQ_WAVES")
-        cells = []
-        for num, den, scale in (("SQ_INSTS_VALU", "SQ_WAVES", 1), ("SQ_INSTS_LDS", "SQ_WAVES", 1),
-                                ("SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES", 4),
-                                ("SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE", 1)):
-            if num in mean and mean.get(den):
-                cells.append("%.3g" % (scale * mean[num] / mean[den]))
-            else:
-                cells.append("-")
-        if waves:
-            derived.append("| %s | " % k + " | ".join(cells) + " |")
+        if not mean.get("SQ_WAVES") or not mean.get("GRBM_GUI_ACTIVE"):
+            continue
+        cycles = mean["GRBM_GUI_ACTIVE"] / xcds
+        mean["_cu_cycles"] = cus * cycles
+        mean["_hits"] = mean.get("TCC_HIT_sum", 0) + mean.get("TCC_MISS_sum", 0)
+        derived.append("| %s | %.0f | %s | %s | %s | %s | %s | %s |" % (
+            k, cycles, ratio(mean, "SQ_INSTS_VALU", "SQ_WAVES"),
+            ratio(mean, "SQ_INSTS_VALU", "_cu_cycles"), ratio(mean, "SQ_LDS_IDX_ACTIVE", "_cu_cycles"),
+            ratio(mean, "SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE"),
+            ratio(mean, "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES"),
+            ratio(mean, "TCC_MISS_sum", "_hits")))
     return out + derived
 
 
@@ -142,6 +156,8 @@ def main(argv=None) -> int:
     t.add_argument("--copy-bytes", type=int, required=True)
     c = sub.add_parser("counters")
     c.add_argument("csvs", nargs="+")
+    c.add_argument("--cus", type=int, default=256)
+    c.add_argument("--xcds", type=int, default=8)
     for p in (s, t, c):
         p.add_argument("--title", default="rocprofv3 summary")
         p.add_argument("--cmd", default="")
@@ -154,7 +170,7 @@ def main(argv=None) -> int:
     elif a.mode == "traffic":
         out += traffic_table(a.fetch_csv, a.write_csv, a.copy_bytes)
     else:
-        out += counter_table(a.csvs)
+        out += counter_table(a.csvs, a.cus, a.xcds)
     print("\n".join(out))
     return 0
 

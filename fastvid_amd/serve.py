@@ -89,6 +89,7 @@ class Request:
     kwargs: dict = field(default_factory=dict)
     seed: int = 0              # encode: seed of the request's frame
     shape: tuple = ()          # encode: (width, height, layout, bit depth)
+    ready: object = None       # encode: event after which the frame is in memory
     stream: object = None      # decode: StoredStream
     luma: int = 0
     done: float = -1.0
@@ -160,7 +161,13 @@ class Server:
                      output=self.pool.view(off + o_out, max(job.output_bytes, 1)),
                      results=self.pool.view(off + o_res, len(batch) * _native.RESULT_BYTES))
         stream = self.streams[slot]
-        stream.wait_stream(torch.cuda.current_stream(self.device))
+        ready = [r.ready for r in batch if r.ready is not None]
+        if ready:
+            # frames come from one in-order stream and a batch holds requests
+            # in arrival order: the last one's event covers every frame
+            stream.wait_event(ready[-1])
+        else:
+            stream.wait_stream(torch.cuda.current_stream(self.device))
         job.launch(stream=stream)
         ev = torch.cuda.Event()
         ev.record(stream)
@@ -185,9 +192,13 @@ class Server:
         sizes = self.pool.view(off + o_sz, 8 * len(batch)).view(torch.int64)
         res = self.pool.view(off + o_res, len(batch) * _native.RESULT_BYTES)
         stream = self.streams[slot]
-        host_sizes = torch.tensor([r.stream.data.numel() for r in batch], dtype=torch.int64)
+        # the sizes go up asynchronously from pinned memory on the slot's
+        # stream (a synchronous copy would stall the request loop); the host
+        # buffer stays referenced by the batch until it retires
+        host_sizes = torch.tensor([r.stream.data.numel() for r in batch], dtype=torch.int64,
+                                  pin_memory=True)
         with torch.cuda.stream(stream):
-            sizes.copy_(host_sizes, non_blocking=False)
+            sizes.copy_(host_sizes, non_blocking=True)
         ins = (ctypes.c_void_p * len(batch))(*[r.stream.data.data_ptr() for r in batch])
         outs = (ctypes.c_void_p * len(batch))(*[o.data_ptr() for o in outputs])
         msg = _native.message_buffer()
@@ -198,7 +209,7 @@ class Server:
             msg, len(msg)), msg)
         ev = torch.cuda.Event()
         ev.record(stream)
-        self.busy[slot] = ("decode", batch, (outputs, res), (off, n), ev)
+        self.busy[slot] = ("decode", batch, (outputs, res, host_sizes), (off, n), ev)
         return True
 
     def _retire(self, slot, now):
@@ -214,9 +225,10 @@ class Server:
                 elif res.status == 0:
                     r.output = int(res.stream_length)
                 r.planes = ()  # the frame is regenerated from its seed if needed again
+                r.ready = None
                 r.done = now
         else:
-            outputs, res_t = job
+            outputs, res_t, _ = job
             host = res_t.cpu().numpy().tobytes()
             for i, r in enumerate(batch):
                 res = _native.Result.from_buffer_copy(host, i * _native.RESULT_BYTES)
@@ -302,14 +314,23 @@ def request_frame(req: "Request", device, stream=None) -> tuple:
     return synthetic.device_frame(req.seed, w, h, layout, bd, device, stream)
 
 
+def _frame_params(plan) -> dict:
+    """Generator parameters of every encode request's frame, by request
+    index (host work done before the clock starts)."""
+    from . import synthetic
+    return {i: synthetic.device_frame_params(rseed, w, h, layout, bd)
+            for i, (t, kind, (w, h, layout, bd, q), rseed) in enumerate(plan) if kind == "encode"}
+
+
 def run(seconds: float = 60.0, rate: float = 2000.0, pool_bytes: int = 64 << 30, slots: int = 2,
         max_batch: int = 32, keep_outputs: bool = False, seed: int = 7, device=None,
         resolutions=None, keep_every: int = 50):
     """Runs the open-loop workload; returns (requests, elapsed seconds, server).
 
     Each encode request's frame is generated on the device from the
-    request's own seed when it arrives (``request_frame``) and dropped when
-    the request completes.  Decode requests replay streams produced earlier:
+    request's own seed when it arrives, on a generator stream of its own
+    (its parameters are drawn on the host before the clock starts; the same
+    planes as ``request_frame``), and dropped when the request completes.  Decode requests replay streams produced earlier:
     one per shape and format encoded before the run, then every
     ``keep_every``-th stream the run produces.  With ``keep_outputs`` every
     output is kept (for ``verify``); otherwise every ``keep_every``-th."""
@@ -318,7 +339,12 @@ def run(seconds: float = 60.0, rate: float = 2000.0, pool_bytes: int = 64 << 30,
     device = torch.device(device) if device is not None else torch.device(
         "cuda", torch.cuda.current_device())
     plan = make_requests(seconds, rate, seed, resolutions=resolutions)
+    params = _frame_params(plan)
     server = Server(device, pool_bytes, slots, max_batch, keep_outputs, keep_every)
+    # frames are generated at arrival on a stream of their own; a batch's
+    # launch waits for the event recorded after its newest frame
+    with torch.cuda.device(device):
+        gen_stream = torch.cuda.Stream(device)
     store = []
     formats = (("gray", 8), ("rgb444", 8), ("gray", 10))
     for k, ((w, h), (layout, bd)) in enumerate((r, f) for r in resolutions for f in formats):
@@ -333,13 +359,15 @@ def run(seconds: float = 60.0, rate: float = 2000.0, pool_bytes: int = 64 << 30,
     nxt = 0
     while True:
         now = clock()
+        arrived = []
         while nxt < len(plan) and plan[nxt][0] <= now:
             t, kind, (w, h, layout, bd, q), rseed = plan[nxt]
             if kind == "encode":
                 req = Request(nxt, "encode", t, kwargs=dict(layout=layout, bit_depth=bd,
                                                             quality=q),
                               luma=w * h, seed=rseed, shape=(w, h, layout, bd))
-                req.planes = request_frame(req, device)
+                req.planes = synthetic.launch_device_frame(params.pop(nxt), device, gen_stream)
+                arrived.append(req)
             else:
                 st = store[rseed % len(store)]
                 req = Request(nxt, "decode", t, stream=st,
@@ -347,6 +375,11 @@ def run(seconds: float = 60.0, rate: float = 2000.0, pool_bytes: int = 64 << 30,
             server.submit(req)
             requests.append(req)
             nxt += 1
+        if arrived:
+            ready = torch.cuda.Event()
+            ready.record(gen_stream)
+            for req in arrived:
+                req.ready = ready
         busy = server.poll(now)
         while server.completed:
             r = server.completed.pop()

@@ -127,6 +127,8 @@ def _ramp_params(rng: np.random.Generator, width: int, height: int):
 
 def _device_params(rng: np.random.Generator, width: int, height: int, bit_depth: int):
     """Ramp, sinusoid and rectangle parameters of one frame (host draws)."""
+    import ctypes
+
     from . import _native
     p = _native.SynthParams()
     p.width, p.height, p.pitch, p.bit_depth = width, height, width, bit_depth
@@ -138,35 +140,19 @@ def _device_params(rng: np.random.Generator, width: int, height: int, bit_depth:
         p.kx[s], p.ky[s] = k * np.cos(direction), k * np.sin(direction)
         p.phase[s] = rng.uniform(0.0, 2.0 * np.pi)
     p.rect_count = 20
-    for r in range(20):
-        p.rects[r][0] = int(rng.integers(0, width))
-        p.rects[r][1] = int(rng.integers(0, height))
-        p.rects[r][2] = int(rng.integers(16, 401))
-        p.rects[r][3] = int(rng.integers(16, 401))
-        p.rects[r][4] = int(rng.integers(0, 256))
+    rects = np.stack([rng.integers(0, width, 20), rng.integers(0, height, 20),
+                      rng.integers(16, 401, 20), rng.integers(16, 401, 20),
+                      rng.integers(0, 256, 20)], axis=1).astype(np.int32)
+    ctypes.memmove(p.rects, rects.ctypes.data, rects.nbytes)
     return p
 
 
-def device_frame(seed: int, width: int, height: int, layout: str, bit_depth: int, device,
-                 stream=None) -> tuple:
-    """A frame of the serving workload generated on the device from `seed`.
-
-    Same formula as ``generate`` (gray: field + noise, then rectangles; sigma
-    3 at 8 bits, 12 at higher depths, in units of the depth) and
-    ``rgb_frame`` (shared field + rectangles, per channel an N(0, 15) offset
-    and N(0, 2) noise), but the noise comes from a counter-based generator on
-    the device, so the frame is not the host generator's frame of the same
-    seed.  Enqueued on `stream` (default: the current stream)."""
-    import ctypes
-
-    import torch
-
+def device_frame_params(seed: int, width: int, height: int, layout: str,
+                        bit_depth: int) -> list:
+    """Generator parameters of each plane of ``device_frame(seed, ...)``."""
     from . import _native
     rng = np.random.default_rng(seed)
     base = _device_params(rng, width, height, bit_depth)
-    if stream is None:
-        stream = torch.cuda.current_stream(device)
-    handle = ctypes.c_void_p(stream.cuda_stream)
     planes = []
     if layout == "rgb444":
         channels = []
@@ -183,35 +169,42 @@ def device_frame(seed: int, width: int, height: int, layout: str, bit_depth: int
         planes.append(base)
     else:
         raise ValueError("device frames are gray or rgb444")
+    return planes
+
+
+def launch_device_frame(params: list, device, stream=None) -> tuple:
+    """Allocates the planes and enqueues their generation on `stream`
+    (default: the current stream); the planes are ready once it reaches
+    this point."""
+    import ctypes
+
+    import torch
+
+    from . import _native
+    if stream is None:
+        stream = torch.cuda.current_stream(device)
+    handle = ctypes.c_void_p(stream.cuda_stream)
     out = []
-    for p in planes:
-        t = torch.empty((height, width), dtype=torch.uint16, device=device)
-        rc = _native.lib().fvh_synth_plane(ctypes.byref(p), ctypes.c_void_p(t.data_ptr()), handle)
-        if rc != 0:
-            raise RuntimeError(f"fvh_synth_plane failed ({rc})")
-        out.append(t)
+    with torch.cuda.stream(stream):
+        for p in params:
+            t = torch.empty((int(p.height), int(p.width)), dtype=torch.uint16, device=device)
+            rc = _native.lib().fvh_synth_plane(ctypes.byref(p), ctypes.c_void_p(t.data_ptr()),
+                                               handle)
+            if rc != 0:
+                raise RuntimeError(f"fvh_synth_plane failed ({rc})")
+            out.append(t)
     return tuple(out)
 
 
-def frame(seed: int, width: int, height: int, layout: str, bit_depth: int) -> tuple:
-    """Host planes of one frame in any layout at any depth (the evaluator
-    workloads): gray is G(seed) with sigma 3 at 8 bits and 12 above; RGB 4:4:4
-    is ``rgb_frame`` rescaled to the depth; YUV 4:2:2 is a luma plane
-    G(seed) and two chroma planes G(seed + 1), G(seed + 2) of width
-    ceil(W/2)."""
-    sigma = 3.0 if bit_depth == 8 else 12.0
-    if layout == "gray":
-        return (generate(seed, width, height, bit_depth, sigma),)
-    if layout == "yuv422":
-        cw = (width + 1) // 2
-        return (generate(seed, width, height, bit_depth, sigma),
-                generate(seed + 1, cw, height, bit_depth, sigma),
-                generate(seed + 2, cw, height, bit_depth, sigma))
-    if layout == "rgb444":
-        planes = rgb_frame(seed, width, height)
-        if bit_depth == 8:
-            return planes
-        maximum = (1 << bit_depth) - 1
-        return tuple(np.clip(np.rint(p.astype(np.float32) * np.float32(maximum / 255.0)), 0,
-                             maximum).astype(np.uint16) for p in planes)
-    raise ValueError(f"unsupported layout {layout!r}")
+def device_frame(seed: int, width: int, height: int, layout: str, bit_depth: int, device,
+                 stream=None) -> tuple:
+    """A frame of the serving workload generated on the device from `seed`.
+
+    Same formula as ``generate`` (gray: field + noise, then rectangles; sigma
+    3 at 8 bits, 12 at higher depths, in units of the depth) and
+    ``rgb_frame`` (shared field + rectangles, per channel an N(0, 15) offset
+    and N(0, 2) noise), but the noise comes from a counter-based generator on
+    the device, so the frame is not the host generator's frame of the same
+    seed.  Enqueued on `stream` (default: the current stream)."""
+    return launch_device_frame(device_frame_params(seed, width, height, layout, bit_depth),
+                               device, stream)
