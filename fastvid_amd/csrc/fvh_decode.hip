@@ -819,6 +819,157 @@ __global__ __launch_bounds__(256) void k_o0_parse(const FrameDesc* __restrict__ 
   }
 }
 
+// Latency form of the table parse, one wave per order-0 shard: the
+// sparse table (table log, symbol count, value deltas and frequencies as
+// canonical LEB128), three lane lengths and four initial states, with the
+// reference's checks; a rejected table sets status 5.  The varints are
+// found 64 bytes at a time: a byte with its high bit clear ends one, so the
+// ballot of such bytes splits the window, and the lane holding a varint's
+// last byte assembles its value from the (at most four) bytes before it.
+// Symbol values and cumulative frequencies are then wave prefix sums over
+// the symbols, 64 at a time, and each symbol's checks run on its own lane.
+// The serial parse stops at its first failing check; every failure gives
+// status 5, so checking everything and rejecting on any failure decides the
+// same.  (The direct-table decoder that follows takes the whole order-0
+// list, so the shards are not filed by alphabet size here.)
+constexpr int kParseVarints = 512;  // table varints of a shard: 2 * 255 at most
+
+// Parses one shard's table into `tab` (nsym 0 when rejected); returns
+// whether it is valid.  Wave-uniform.
+__device__ __forceinline__ bool parse_o0_table(const uint8_t* body, int L, uint32_t max_folded,
+                                              O0Tab* tab, uint32_t* V, uint16_t* E) {
+  const int lane = lane_id();
+  bool bad = L < 1 || body[0] != kRansTableLog;
+  int base = 1, nvar = 0, need = kParseVarints, nsym = 0;
+  uint32_t first_bad = 0xffffffffu;  // lowest index of a malformed varint
+  while (!bad && nvar < need) {
+    const int p = base + lane;
+    const bool in = p < L;
+    const uint32_t b = in ? (uint32_t)body[p] : 0x80u;
+    const uint64_t T = ballot(in && (b & 0x80u) == 0);
+    // no varint ends in the window: one longer than five bytes, or the body
+    // ends inside one
+    if (T == 0) {
+      bad = true;
+      break;
+    }
+    const uint64_t below = T & ((1ull << lane) - 1ull);
+    const int start = below ? 64 - __clzll((long long)below) : 0;
+    const int seg = lane - start + 1;
+    uint32_t v = b & 0x7fu, prev = b;
+#pragma unroll
+    for (int dd = 1; dd < 5; ++dd) {
+      prev = from_prev_lane(prev);
+      if (dd < seg) v = (v << 7) | (prev & 0x7fu);
+    }
+    // (assembled most significant byte first: the varint's last byte is the
+    // most significant 7 bits)
+    const bool term = (T >> lane) & 1ull;
+    const int idx = nvar + popc64(below);
+    const bool ok = seg <= 5 && (seg < 5 || b <= 0x0fu) && varint_len(v) == seg;
+    if (term && idx < kParseVarints) {
+      V[idx] = v;
+      E[idx] = (uint16_t)(p + 1);
+    }
+    first_bad = min(first_bad, wave_min(term && !ok ? (uint32_t)idx : 0xffffffffu));
+    nvar += popc64(T);
+    base += 64 - __clzll((long long)T);  // past the window's last terminator
+    if (need == kParseVarints) {
+      __builtin_amdgcn_wave_barrier();
+      nsym = (int)V[0];
+      if (first_bad == 0 || nsym == 0 || nsym > 255) bad = true;
+      need = 2 * nsym;
+    }
+    if (first_bad < (uint32_t)need) bad = true;
+  }
+  if (!bad && first_bad < (uint32_t)need) bad = true;
+  __builtin_amdgcn_wave_barrier();
+  int cursor = 0;
+  uint32_t carry_value = 0, carry_assigned = 0;
+  if (!bad) {
+    cursor = E[need - 1];
+    for (int k0 = 0; k0 < nsym; k0 += 64) {
+      const int k = k0 + lane;
+      const bool valid = k < nsym;
+      const uint32_t delta = valid ? V[1 + 2 * k] : 0u;
+      const bool has_freq = valid && k + 1 < nsym;
+      // frequencies past 4096 fail below either way: clamped, the sums
+      // stay in 32 bits
+      const uint32_t fr_in = has_freq ? min(V[2 + 2 * k], 4097u) : 0u;
+      const uint32_t value = carry_value + wave_incl_scan(delta);
+      const uint32_t incl = wave_incl_scan(fr_in);
+      const uint32_t assigned = carry_assigned + incl - fr_in;  // before symbol k
+      const uint32_t fr = has_freq ? fr_in : (1u << kRansTableLog) - assigned;
+      const bool fail = valid && ((k != 0 && delta == 0) || value > max_folded || fr == 0 ||
+                                  assigned + fr > (1u << kRansTableLog) ||
+                                  (has_freq && assigned + fr == (1u << kRansTableLog)));
+      if (valid) tab->ent[k] = assigned | (value << 13);
+      if (wave_or(fail ? 1u : 0u)) bad = true;
+      carry_value = (uint32_t)__builtin_amdgcn_readlane((int)value, 63);
+      carry_assigned += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    }
+  }
+  // three lane lengths and four states, little-endian u32s
+  if (!bad && cursor + 28 > L) bad = true;
+  if (!bad) {
+    uint32_t w = 0;
+    if (lane < 7) {
+      const uint8_t* q = body + cursor + 4 * lane;
+      w = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16) | ((uint32_t)q[3] << 24);
+    }
+    const uint32_t w0 = shfl(w, 0), w1 = shfl(w, 1), w2 = shfl(w, 2);
+    const int at0 = cursor + 28;
+    const int len0 = (int)w0, len1 = (int)w1, len2 = (int)w2;
+    const int len3 = (int)((uint32_t)(L - at0) - (w0 + w1 + w2));
+    const int64_t at = (int64_t)at0 + len0 + len1 + len2 + len3;
+    const bool state_ok = lane < 3 || lane >= 7 || w >= kRansL;
+    const uint32_t state = shfl(w, 3 + (lane & 3));  // (every lane takes part in the shuffle)
+    if (len0 < 0 || len1 < 0 || len2 < 0 || len3 < 0 || at != L || !wave_min(state_ok ? 1u : 0u)) {
+      bad = true;
+    } else if (lane < 4) {
+      const int len = lane == 0 ? len0 : lane == 1 ? len1 : lane == 2 ? len2 : len3;
+      const int start = at0 + (lane > 0 ? len0 : 0) + (lane > 1 ? len1 : 0) + (lane > 2 ? len2 : 0);
+      tab->start[lane] = start;
+      tab->length[lane] = len;
+      tab->state[lane] = state;
+    }
+  }
+  if (bad) {
+    if (lane == 0) tab->nsym = 0;
+    return false;
+  }
+  if (lane == 0) {
+    tab->ent[nsym] = 1u << kRansTableLog;
+    tab->nsym = nsym;
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(64) void k_o0_parse_wave(const FrameDesc* __restrict__ frames,
+                                                     FrameState* __restrict__ states,
+                                                     const DShard* __restrict__ dshards,
+                                                     const int32_t* __restrict__ lists,
+                                                     const int32_t* __restrict__ list_count,
+                                                     O0Tab* __restrict__ tabs, char* gbase) {
+  __shared__ uint32_t value[kParseVarints];
+  __shared__ uint16_t end[kParseVarints];
+  const int item = (int)blockIdx.x;
+  if (item >= list_count[0]) return;
+  const DShard d = dshards[lists[item]];
+  O0Tab* tab = tabs + item;
+  FrameState* st = &states[d.frame];
+  if (st->status != 0) {
+    if (lane_id() == 0) tab->nsym = 0;
+    return;
+  }
+  const FrameDesc& f = frames[d.frame];
+  const uint8_t* body = as_global(gbase, (const uint8_t*)f.stream + d.body_offset);
+  if (!parse_o0_table(body, d.body_length, (uint32_t)f.max_sample * 2u, tab, value, end) &&
+      lane_id() == 0) {
+    set_status(st, 5);
+  }
+}
+
 // Order-0 rANS lanes (decode.cu:511-549): sixteen shards per wave, one quad
 // of lanes per shard, lane r of the quad running rANS lane r.  Waves are
 // persistent and take groups of 16 shards from one of three tier lists, by
@@ -1615,18 +1766,25 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
                        b.states, b.tiles, b.tile_total, b.dshards, b.lists, b.list_count,
                        b.shard_total, (char*)b.folded);
   }
-  {
+  // a batch whose shards all fit the device at once takes the latency forms
+  // of the table parse (a wave per shard) and of the order-0 decoder (four
+  // shards per wave, direct slot tables): a frame's decode time is then one
+  // shard's chain of steps, each one LDS read long
+  static const int direct_per_cu = blocks_per_cu((const void*)k_dec_o0_direct<FT>, 0);
+  const bool latency_form = b.shard_total <= kO0DirectShards * direct_per_cu * device_cus();
+  int32_t* const tier_lists = b.lists + 2 * b.shard_total;
+  if (latency_form) {
+    FVH_TIMED("k_o0_parse_wave", stream);
+    hipLaunchKernelGGL(k_o0_parse_wave, dim3(max(b.shard_total, 1)), dim3(64), 0, stream,
+                       b.frames, b.states, b.dshards, b.lists, b.list_count, b.o0tabs,
+                       (char*)b.folded);
+  } else {
     FVH_TIMED("k_o0_parse", stream);
     hipLaunchKernelGGL(k_o0_parse, dim3((b.shard_total + 255) / 256), dim3(256), 0, stream,
-                       b.frames, b.states, b.dshards, b.lists, b.list_count, b.lists + 2 * b.shard_total,
+                       b.frames, b.states, b.dshards, b.lists, b.list_count, tier_lists,
                        b.shard_total, b.o0tabs, (char*)b.folded);
   }
-  // a batch whose shards all fit the device at once in the latency form
-  // (four shards per wave, direct slot tables) takes it: a frame's decode
-  // time is then one shard's chain of steps, each one LDS read long
-  static const int direct_per_cu = blocks_per_cu((const void*)k_dec_o0_direct<FT>, 0);
-  const int direct_cap = kO0DirectShards * direct_per_cu * device_cus();
-  if (b.shard_total <= direct_cap) {
+  if (latency_form) {
     FVH_TIMED("k_dec_o0_direct", stream);
     const int blocks = (b.shard_total + kO0DirectShards - 1) / kO0DirectShards;
     hipLaunchKernelGGL((k_dec_o0_direct<FT>), dim3(max(blocks, 1)), dim3(64), 0, stream, b.frames,

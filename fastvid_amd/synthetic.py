@@ -208,3 +208,27 @@ def device_frame(seed: int, width: int, height: int, layout: str, bit_depth: int
     seed.  Enqueued on `stream` (default: the current stream)."""
     return launch_device_frame(device_frame_params(seed, width, height, layout, bit_depth),
                                device, stream)
+
+
+def frame(seed: int, width: int, height: int, layout: str, bit_depth: int) -> tuple:
+    """Host planes of one frame in any layout at any depth (the evaluator
+    workloads): gray is G(seed) with sigma 3 at 8 bits and 12 above; RGB 4:4:4
+    is ``rgb_frame`` rescaled to the depth; YUV 4:2:2 is a luma plane
+    G(seed) and two chroma planes G(seed + 1), G(seed + 2) of width
+    ceil(W/2)."""
+    sigma = 3.0 if bit_depth == 8 else 12.0
+    if layout == "gray":
+        return (generate(seed, width, height, bit_depth, sigma),)
+    if layout == "yuv422":
+        cw = (width + 1) // 2
+        return (generate(seed, width, height, bit_depth, sigma),
+                generate(seed + 1, cw, height, bit_depth, sigma),
+                generate(seed + 2, cw, height, bit_depth, sigma))
+    if layout == "rgb444":
+        planes = rgb_frame(seed, width, height)
+        if bit_depth == 8:
+            return planes
+        maximum = (1 << bit_depth) - 1
+        return tuple(np.clip(np.rint(p.astype(np.float32) * np.float32(maximum / 255.0)), 0,
+                             maximum).astype(np.uint16) for p in planes)
+    raise ValueError(f"unsupported layout {layout!r}")

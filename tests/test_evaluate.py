@@ -38,3 +38,17 @@ def test_distribution_and_samples():
     assert ids["4k-yuv422-10-x24"].plane_shapes == ((2160, 3840), (2160, 1920), (2160, 1920))
     assert ids["4k-rgb444-10-x24"].batch_frames == 24
     assert ids["1080p-rgb444-10"].raw_bytes_per_frame == 1920 * 1080 * 3 * 2
+
+
+def test_evaluator_frames_have_the_sample_shapes():
+    import numpy as np
+
+    from fastvid_amd import synthetic
+    for fmt, bd in (("yuv422", 10), ("rgb444", 10), ("rgb444", 8), ("gray", 12)):
+        planes = synthetic.frame(7, 65, 33, fmt, bd)
+        cw = 33 if fmt == "yuv422" else 65
+        shapes = [(33, 65)] + ([(33, cw)] * 2 if fmt != "gray" else [])
+        assert [p.shape for p in planes] == shapes
+        assert all(p.dtype == np.uint16 and int(p.max()) < (1 << bd) for p in planes)
+        again = synthetic.frame(7, 65, 33, fmt, bd)
+        assert all(np.array_equal(a, b) for a, b in zip(planes, again))
