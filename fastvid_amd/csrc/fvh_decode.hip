@@ -977,15 +977,18 @@ __global__ __launch_bounds__(64) void k_o0_parse_wave(const FrameDesc* __restric
 // shard), so the per-shard tables in LDS are sized by the alphabet and small
 // alphabets leave room for more waves on a compute unit.
 //
-// Slot -> rank: the 4096 slots form 128 buckets of 32; mask[b] has bit j set
-// where a symbol's slot range starts at slot 32 b + j (the first symbol
-// excluded) and prefix[b] counts the starts before the bucket, so the rank of
-// a slot is prefix[b] + popcount(mask[b] & ((2 << j) - 1)).  Rank -> symbol:
-// the pair {cum_k | (freq_k - 4096 mod 2^13) << 13, value_k}, one 8-byte LDS
-// read.  The state update freq*(x >> 12) + (x & 4095) - cum is computed as
-// (freq - 4096)*(x >> 12) + (x - cum), one signed 24-bit multiply-add whose
-// low 32 bits equal the reference's (exact) u32 result.  Tables: 640 bytes +
-// 8 per symbol, against the reference's 4096-byte slot table
+// Slot -> rank: the 4096 slots form buckets of S = 32 - P slots, P the bits
+// of the largest rank of the tier (4, 6, 8 for 16, 64, 255 symbols); the
+// bucket's word has bit j set where a symbol's slot range starts at slot
+// S b + j (the first symbol excluded) and holds in its top P bits the count
+// of starts before the bucket, so the rank of slot s is one LDS read and a
+// popcount: (w >> S) + popcount(w & ((2 << j) - 1)), b = floor(s / S) by a
+// 24-bit multiply.  Rank -> symbol: the pair {cum_k | (freq_k - 4096 mod
+// 2^13) << 13, value_k}, one 8-byte LDS read.  The state update
+// freq*(x >> 12) + (x & 4095) - cum is computed as (freq - 4096)*(x >> 12) +
+// (x - cum), one signed 24-bit multiply-add whose low 32 bits equal the
+// reference's (exact) u32 result.  Tables: 592 bytes (16 symbols) to 688
+// (255) + 8 per symbol, against the reference's 4096-byte slot table
 // (decode.cu:519-527).
 //
 // The symbol loop runs in rounds of four steps with uniform control flow: a
@@ -999,7 +1002,25 @@ __global__ __launch_bounds__(64) void k_o0_parse_wave(const FrameDesc* __restric
 // its end consumes more bytes than it has, so the final check "bytes consumed
 // == lane length and state == 2^23" covers every failure of the reference.
 constexpr int kO0Shards = 16;
-constexpr int kO0Buckets = 128;
+
+// Slot buckets of a tier whose ranks are below kMax <= 255: kPrefixBits for
+// the count of symbol starts before a bucket, kSlots slot bits.
+template <int kMax>
+struct O0Buckets {
+  static constexpr int kPrefixBits = kMax <= 16 ? 4 : kMax <= 64 ? 6 : 8;
+  static constexpr int kSlots = 32 - kPrefixBits;              // 28, 26, 24
+  static constexpr int kCount = (4096 + kSlots - 1) / kSlots;  // 147, 158, 171
+  static constexpr int kWords = (kCount + 3) & ~3;             // whole quads
+  static constexpr int kPerLane = kWords / 4;                  // prefix pass
+  // floor(s / kSlots) == (s * kMagic) >> 16 for every slot s < 4096:
+  // kMagic = ceil(2^16 / kSlots) exceeds 2^16 / kSlots by e / kSlots with
+  // e = kMagic * kSlots - 2^16, and s * e < 2^16 keeps the error below the
+  // gap to the next multiple
+  static constexpr uint32_t kMagic = (65536u + kSlots - 1) / kSlots;
+  static_assert((kMagic * kSlots - 65536u) * 4095u < 65536u, "bucket division");
+  static_assert(kMax - 1 < (1 << kPrefixBits), "prefix count width");
+  __device__ __forceinline__ static uint32_t of(uint32_t s) { return __umul24(s, kMagic) >> 16; }
+};
 
 // A 16-byte chunk as two little-endian quadwords.  (Chunks are kept as
 // scalars: a conditional between two vector-typed objects would be compiled
@@ -1062,26 +1083,27 @@ struct O0Block {
 
 template <typename FT, int kMax>
 struct O0Lds {
-  uint32_t mask[kO0Shards][kO0Buckets];
+  uint32_t bucket[kO0Shards][O0Buckets<kMax>::kWords];
   uint2 row[kO0Shards][O0Row<kMax>::kPairs];
-  uint8_t prefix[kO0Shards][kO0Buckets];
   uint32_t stage[kO0Shards][O0Block<FT>::kWords];
 };
 
-// Slot -> symbol of the bucket tables (k_dec_o0): slot -> rank by the
-// boundary mask and prefix count of the slot's 32-slot bucket, rank -> the
+// Slot -> symbol of the bucket tables (k_dec_o0): slot -> rank by the word
+// of the slot's bucket (start bits and the count before it), rank -> the
 // {cum | (freq - 4096) << 13, value} pair; returns the decoded state
 // freq * (x >> 12) + (x & 4095) - cum before renormalisation, as one signed
 // 24-bit multiply-add (its low 32 bits equal the reference's u32 result).
 template <typename FT, int kMax>
 struct BucketLookup {
-  const uint32_t* mask;
-  const uint8_t* prefix;
+  const uint32_t* bucket;
   const uint2* ent;
   __device__ __forceinline__ uint32_t update(uint32_t x, uint32_t& value) const {
-    const uint32_t b = (x >> 5) & (kO0Buckets - 1);
-    const uint32_t m = mask[b] & ((2u << (x & 31u)) - 1u);
-    const uint32_t rank = bcnt_add(m, prefix[b]);
+    using B = O0Buckets<kMax>;
+    const uint32_t s = x & 4095u;
+    const uint32_t b = B::of(s);
+    const uint32_t w = bucket[b];
+    const uint32_t j = s - (uint32_t)B::kSlots * b;
+    const uint32_t rank = bcnt_add(w & ((2u << j) - 1u), w >> B::kSlots);
     const uint2 ev = ent[rank];
     value = ev.y;
     const uint32_t c = ev.x & 0x1fffu;
@@ -1269,9 +1291,9 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
   const int g = lane >> 2;
   const int r = lane & 3;
   const int count = *tier_count;
-  uint32_t* const mask = lds.mask[g];
+  using B = O0Buckets<kMax>;
+  uint32_t* const bucket = lds.bucket[g];
   uint2* const ent = lds.row[g];
-  uint8_t* const prefix = lds.prefix[g];
   FT* const stage = (FT*)lds.stage[g];
   // a wave's first group is its block index, later ones come from the
   // queue: blocks past the list's end (all of them, for an empty tier) leave
@@ -1282,50 +1304,41 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
     const int item = at < count ? tier_list[at] : -1;
     const O0Tab* tab = tabs + (item >= 0 ? item : 0);
     const int nsym = item >= 0 ? tab->nsym : 0;
-    // ---- tables: entries and values, boundary masks, prefix counts
-#pragma unroll
-    for (int j = 0; j < 8; ++j) ((uint4*)mask)[8 * r + j] = make_uint4(0, 0, 0, 0);
+    // ---- tables: entries and values, bucket start bits, counts before
+    // each bucket (a parsed table has 1 <= cum_k < 4096 for k >= 1)
+    for (int q = r; q < B::kWords / 4; q += 4) ((uint4*)bucket)[q] = make_uint4(0, 0, 0, 0);
     __builtin_amdgcn_wave_barrier();
     for (int k = r; k < nsym; k += 4) {
       const uint32_t e0 = tab->ent[k];
       const uint32_t c = e0 & 0x1fffu;
       const uint32_t fr = (tab->ent[k + 1] & 0x1fffu) - c;
       ent[k] = make_uint2(c | (((fr - 4096u) & 0x1fffu) << 13), e0 >> 13);
-      if (k > 0) atomicOr(&mask[c >> 5], 1u << (c & 31u));
+      if (k > 0) {
+        const uint32_t b = B::of(c);
+        atomicOr(&bucket[b], 1u << (c - (uint32_t)B::kSlots * b));
+      }
     }
     __builtin_amdgcn_wave_barrier();
     {
-      // lane r: buckets 32 r .. 32 r + 31
-      uint32_t cnt[32];
+      // lane r: buckets kPerLane r .. kPerLane (r + 1) - 1
+      uint32_t* const mine = bucket + B::kPerLane * r;
       uint32_t total = 0;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const uint4 w = ((const uint4*)mask)[8 * r + j];
-        cnt[4 * j + 0] = total;
-        total += __popc(w.x);
-        cnt[4 * j + 1] = total;
-        total += __popc(w.y);
-        cnt[4 * j + 2] = total;
-        total += __popc(w.z);
-        cnt[4 * j + 3] = total;
-        total += __popc(w.w);
-      }
+      for (int i = 0; i < B::kPerLane; ++i) total += __popc(mine[i]);
       // exclusive scan of the four lane totals of the quad
       const uint32_t t1 = __shfl_up(total, 1, 4);
       const uint32_t s1 = total + (r >= 1 ? t1 : 0u);
       const uint32_t t2 = __shfl_up(s1, 2, 4);
       const uint32_t incl = s1 + (r >= 2 ? t2 : 0u);
-      const uint32_t base = incl - total;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const uint32_t w = (base + cnt[4 * j]) | ((base + cnt[4 * j + 1]) << 8) |
-                           ((base + cnt[4 * j + 2]) << 16) | ((base + cnt[4 * j + 3]) << 24);
-        ((uint32_t*)prefix)[8 * r + j] = w;
+      uint32_t run = incl - total;
+      for (int i = 0; i < B::kPerLane; ++i) {
+        const uint32_t w = mine[i];
+        mine[i] = w | (run << B::kSlots);
+        run += __popc(w);
       }
     }
     __builtin_amdgcn_wave_barrier();
     // ---- the shard's rANS lanes
-    BucketLookup<FT, kMax> look{mask, prefix, ent};
+    BucketLookup<FT, kMax> look{bucket, ent};
     o0_lanes<FT>(look, item, nsym, tab, r, lane, frames, dshards, lists, states, folded, stage,
                  gbase, sink, tabs);
     int next = 0;
