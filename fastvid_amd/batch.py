@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _native
-from .codec import EncodeJob, _read_header, _read_results, frame_args
+from .codec import EncodeJob, _pinned_results, _read_header, _read_results, frame_args
 
 
 @dataclass(frozen=True)
@@ -167,6 +167,7 @@ class GraphDecoder:
                                         device=self.device) for i in range(n)]
             self.results = torch.empty(n * _native.RESULT_BYTES, dtype=torch.uint8,
                                        device=self.device)
+            self._host_results = _pinned_results(n)
             self._ins = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.streams])
             self._outs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.outputs])
             self._frames = [self.planes(i) for i in range(n)]
@@ -223,7 +224,7 @@ class GraphDecoder:
         return tuple(planes)
 
     def finish(self) -> list:
-        status = _read_results(self.results, self.stream)["status"]
+        status = _read_results(self.results, self.stream, self._host_results)["status"]
         bad = np.flatnonzero(status)
         if bad.size:
             first = int(status[bad[0]])

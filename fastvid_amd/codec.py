@@ -25,15 +25,25 @@ _RESULT_DTYPE = np.dtype([("stream_length", "<i8"), ("status", "<i4"), ("flags",
 assert _RESULT_DTYPE.itemsize == _native.RESULT_BYTES
 
 
-def _read_results(results: torch.Tensor, stream) -> np.ndarray:
-    """Reads the result records back with one synchronous copy on the stream
-    the job ran on (so it waits for that stream only) and returns them as
-    records.  (A non-blocking copy into pinned memory plus a stream wait took
-    20 us longer per one-frame call.)"""
+def _pinned_results(count: int) -> torch.Tensor:
+    """A pinned host buffer for `count` result records."""
+    return torch.empty(count * _native.RESULT_BYTES, dtype=torch.uint8, pin_memory=True)
+
+
+def _read_results(results: torch.Tensor, stream, host: torch.Tensor = None) -> np.ndarray:
+    """Copies the result records into the pinned buffer `host` on the stream
+    the job ran on, waits for that stream only, and returns them as records
+    (a view of `host`, valid until its next read).  On 16 4K frames this takes
+    10 us where a synchronous `.cpu()` of the records took 33."""
     if stream is None:
         stream = torch.cuda.current_stream(results.device)
-    with torch.cuda.stream(stream):
-        host = results.cpu()
+    if host is None:
+        host = _pinned_results(results.numel() // _native.RESULT_BYTES)
+    msg = _native.message_buffer()
+    _native.check(_native.lib().fvh_results_read(
+        ctypes.c_void_p(results.data_ptr()), ctypes.c_void_p(host.data_ptr()),
+        results.numel() // _native.RESULT_BYTES, ctypes.c_void_p(stream.cuda_stream), msg,
+        len(msg)), msg)
     return host.numpy().view(_RESULT_DTYPE)
 
 _LAYOUTS = {"gray": 0, "yuv422": 1, "rgb444": 2}
@@ -177,6 +187,8 @@ class EncodeJob:
             max(self.output_bytes, 1), dtype=torch.uint8, device=self.device)
         self.results = results if results is not None else torch.empty(
             n * _native.RESULT_BYTES, dtype=torch.uint8, device=self.device)
+        self._host_results = _pinned_results(n)
+        self._views = None  # (stream lengths, their views of the output)
         return self
 
     def device_buffers(self) -> list:
@@ -198,7 +210,7 @@ class EncodeJob:
         return self
 
     def finish(self):
-        rec = _read_results(self.results, self._stream)
+        rec = _read_results(self.results, self._stream, self._host_results)
         bad = np.flatnonzero(rec["status"])
         if bad.size:
             status = int(rec["status"][bad[0]])
@@ -207,10 +219,19 @@ class EncodeJob:
             if status == 100:
                 raise RuntimeError("tile payload is too large")
             raise RuntimeError(f"encoder failed (status {status})")
+        lengths = rec["stream_length"].copy()
+        # the views of an earlier call whose streams had the same lengths are
+        # the same views of the output
+        if self._views is not None and np.array_equal(self._views[0], lengths):
+            return list(self._views[1])
+        views = self._stream_views(lengths)
+        self._views = (lengths, views)
+        return list(views)
+
+    def _stream_views(self, lengths: np.ndarray) -> list:
         # Each stream starts at its frame's reserved offset: one split of the output
         # into [gap, stream, gap, stream, ..., tail] yields every stream view at once.
         starts = np.frombuffer(self.offsets, dtype=np.uint64).astype(np.int64)
-        lengths = rec["stream_length"]
         ends = starts + lengths
         gaps = starts - np.concatenate(([0], ends[:-1]))
         tail = self.output.numel() - int(ends[-1])
@@ -326,6 +347,7 @@ class DecodeJob:
         self._outs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.outputs])
         # The plane views of every output are fixed once the outputs are.
         self._frames = [self.planes(i) for i in range(n)]
+        self._host_results = _pinned_results(n)
         return self
 
     def device_buffers(self) -> list:
@@ -374,7 +396,7 @@ class DecodeJob:
         return tuple(planes)
 
     def finish(self):
-        rec = _read_results(self.results, self._stream)
+        rec = _read_results(self.results, self._stream, self._host_results)
         bad = np.flatnonzero(rec["status"])
         if bad.size:
             raise RuntimeError(

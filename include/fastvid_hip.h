@@ -143,6 +143,13 @@ int fvh_decode_launch_sized(const fvh_stream_info* infos, const void* const* str
                             void* workspace, uint64_t workspace_bytes, fvh_result* results,
                             int32_t wavefront, fvh_stream_t stream, char* msg, size_t msg_len);
 
+/* Copies the `count` result records of a job at `results` (device) to
+ * `host` (pinned host memory) on the job's stream and waits for that stream:
+ * a job's only device-to-host transfer, as the reference's `.cpu()` of its
+ * status and analysis tensors (cuda/csrc/encode.cu:806-807, decode.cu:718). */
+int fvh_results_read(const fvh_result* results, fvh_result* host, int32_t count,
+                     fvh_stream_t stream, char* msg, size_t msg_len);
+
 /* Per-kernel timing (no counterpart in the reference, which is timed from
  * outside with CUDA events around whole calls, scripts/evaluate.py:447-463).
  * While enabled, every kernel launch of the library is bracketed by two
