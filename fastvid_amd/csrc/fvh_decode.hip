@@ -983,8 +983,9 @@ __global__ __launch_bounds__(64) void k_o0_parse_wave(const FrameDesc* __restric
 // S b + j (the first symbol excluded) and holds in its top P bits the count
 // of starts before the bucket, so the rank of slot s is one LDS read and a
 // popcount: (w >> S) + popcount(w & ((2 << j) - 1)), b = floor(s / S) by a
-// 24-bit multiply.  Rank -> symbol: the pair {cum_k | (freq_k - 4096 mod
-// 2^13) << 13, value_k}, one 8-byte LDS read.  The state update
+// 24-bit multiply.  Rank -> symbol: one 8-byte LDS read of the pair
+// {cum_k | value_k << 16, freq_k - 4096} (16-bit samples) or {cum_k |
+// (freq_k - 4096 mod 2^13) << 13, value_k} (32-bit).  The state update
 // freq*(x >> 12) + (x & 4095) - cum is computed as (freq - 4096)*(x >> 12) +
 // (x - cum), one signed 24-bit multiply-add whose low 32 bits equal the
 // reference's (exact) u32 result.  Tables: 592 bytes (16 symbols) to 688
@@ -1105,10 +1106,17 @@ struct BucketLookup {
     const uint32_t j = s - (uint32_t)B::kSlots * b;
     const uint32_t rank = bcnt_add(w & ((2u << j) - 1u), w >> B::kSlots);
     const uint2 ev = ent[rank];
-    value = ev.y;
-    const uint32_t c = ev.x & 0x1fffu;
-    const int fneg = ((int)(ev.x << 6)) >> 19;  // freq - 4096, sign-extended
-    return (uint32_t)__mul24(fneg, (int)(x >> kRansTableLog)) + (x - c);
+    if constexpr (sizeof(FT) == 2) {
+      // {cum | value << 16, freq - 4096}: the subtraction reads cum in
+      // place, and the stage takes the value's half as it is
+      value = ev.x >> 16;
+      return (uint32_t)__mul24((int)ev.y, (int)(x >> kRansTableLog)) + (x - (ev.x & 0xffffu));
+    } else {
+      value = ev.y;
+      const uint32_t c = ev.x & 0x1fffu;
+      const int fneg = ((int)(ev.x << 6)) >> 19;  // freq - 4096, sign-extended
+      return (uint32_t)__mul24(fneg, (int)(x >> kRansTableLog)) + (x - c);
+    }
   }
 };
 
@@ -1312,7 +1320,11 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
       const uint32_t e0 = tab->ent[k];
       const uint32_t c = e0 & 0x1fffu;
       const uint32_t fr = (tab->ent[k + 1] & 0x1fffu) - c;
-      ent[k] = make_uint2(c | (((fr - 4096u) & 0x1fffu) << 13), e0 >> 13);
+      if constexpr (sizeof(FT) == 2) {
+        ent[k] = make_uint2(c | ((e0 >> 13) << 16), fr - 4096u);
+      } else {
+        ent[k] = make_uint2(c | (((fr - 4096u) & 0x1fffu) << 13), e0 >> 13);
+      }
       if (k > 0) {
         const uint32_t b = B::of(c);
         atomicOr(&bucket[b], 1u << (c - (uint32_t)B::kSlots * b));

@@ -422,6 +422,11 @@ struct Chunk8<uint16_t> {
     const uint32_t w = b < 2 ? v.x : b < 4 ? v.y : b < 6 ? v.z : v.w;
     return (b & 1) ? (w >> 16) : (w & 0xffffu);
   }
+  // low bytes of symbols r + 4 and r as bytes 0 and 1 (r < 4)
+  __device__ __forceinline__ uint32_t low_pair(int r) const {
+    return __builtin_amdgcn_perm(r < 2 ? v.x : v.y, r < 2 ? v.z : v.w,
+                                 (r & 1) ? 0x0c0c0602u : 0x0c0c0400u);
+  }
   static __device__ __forceinline__ p16 pk(uint32_t w) { return __builtin_bit_cast(p16, w); }
   static __device__ __forceinline__ uint32_t un(p16 w) { return __builtin_bit_cast(uint32_t, w); }
   // histogram atomics (zeros to the junk slot) and the zero bits: bit b
@@ -481,6 +486,10 @@ struct Chunk8<uint32_t> {
       case 0: return a.x; case 1: return a.y; case 2: return a.z; case 3: return a.w;
       case 4: return c.x; case 5: return c.y; case 6: return c.z; default: return c.w;
     }
+  }
+  // low bytes of symbols r + 4 and r as bytes 0 and 1 (r < 4)
+  __device__ __forceinline__ uint32_t low_pair(int r) const {
+    return __builtin_amdgcn_perm(get(r), get(r + 4), 0x0c0c0400u);
   }
   __device__ __forceinline__ uint32_t hist_add(uint32_t* row0, uint32_t* row1, uint32_t junk) const {
     const uint32_t j = junk & 0xffffu;
@@ -741,17 +750,31 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
     for (int c = 0; c < 8; ++c) ch[c].launder();
     uint8_t* ranks = rank_area + (int64_t)(s / kRansShards) * kRankGroupBytes +
                      16 * 4 * (s % kRansShards);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    // (non-temporal: the rank area, a byte per symbol, is read once, by
+    // k_rans, which on 16 4K RGB frames then takes 5 % less time)
+    auto put = [&](int c, int r, uint32_t pair) __attribute__((always_inline)) {
       const int p = 1022 - (128 * c + 2 * lane);  // byte of symbol b = r + 4 (lane r)
       uint8_t* const at = ranks + 1024 * (p >> 4) + (p & 15);
+      __builtin_nontemporal_store((uint16_t)pair, (uint16_t*)(at + 16 * r));
+    };
+    if (distinct == (int)maxv + 1) {
+      // every value up to the largest is present: a value's rank is the
+      // value, and a pair of rank bytes is one byte permute of the symbols
+      // (the symbols past a partial shard's end are zeros either way)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const uint32_t rk0 = lds.rank_of[ch[c].get(r)];
-        const uint32_t rk1 = lds.rank_of[ch[c].get(r + 4)];
-        // non-temporal: the rank area (a byte per symbol) is read once, by
-        // k_rans, which on 16 4K RGB frames then takes 5 % less time
-        __builtin_nontemporal_store((uint16_t)(rk1 | (rk0 << 8)), (uint16_t*)(at + 16 * r));
+      for (int c = 0; c < 8; ++c) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) put(c, r, ch[c].low_pair(r));
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t rk0 = lds.rank_of[ch[c].get(r)];
+          const uint32_t rk1 = lds.rank_of[ch[c].get(r + 4)];
+          put(c, r, rk1 | (rk0 << 8));
+        }
       }
     }
   }
