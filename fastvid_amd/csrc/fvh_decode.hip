@@ -1179,7 +1179,10 @@ __device__ __forceinline__ void o0_lanes(const Lookup& look, int item, int nsym,
     stage[4 * slot_pos + r] = (FT)value;  // slot_pos: step within the block
     // one byte below 2^23, two below 2^15 (never more: xd >= 2^11): the
     // leading zeros less one, rounded down to a multiple of 8 (none from
-    // 2^31, where a state read from a malformed stream may start)
+    // 2^31, where a state read from a malformed stream may start).  (Two
+    // compares and two selects over both renormalised candidates make a
+    // shorter chain on paper but run 9 % slower: a select waits longer for
+    // a compare's lane mask than for a VGPR.)
     const uint32_t lz = (uint32_t)__builtin_clz(xd);
     int sh = (int)(__builtin_elementwise_sub_sat(lz, 1u) & 0x18u);
     if (!live) sh = 0;
@@ -1191,10 +1194,14 @@ __device__ __forceinline__ void o0_lanes(const Lookup& look, int item, int nsym,
   // the block's symbols (from step s0): lane r stores kPer of them from
   // symbol kPer * r, 32 bytes
   constexpr int kPer = O0Block<FT>::kPerLane;
+  // (non-temporal: the residuals are read once, by the reconstruction, and
+  // streaming them past the L2 keeps the lanes' body lines there between
+  // their chunk loads)
   auto flush_full = [&](int s0) __attribute__((always_inline)) {
+    using q4 = uint32_t __attribute__((ext_vector_type(4)));
     FT* const dst = flush_base + (4 * s0 + kPer * r) * flush_step;
-    ((uint4*)dst)[0] = ((const uint4*)(stage + kPer * r))[0];
-    ((uint4*)dst)[1] = ((const uint4*)(stage + kPer * r))[1];
+    __builtin_nontemporal_store(((const q4*)(stage + kPer * r))[0], (q4*)dst);
+    __builtin_nontemporal_store(((const q4*)(stage + kPer * r))[1], (q4*)dst + 1);
   };
   auto flush = [&](int s0) __attribute__((always_inline)) {
     const int p0 = 4 * s0 + kPer * r;
