@@ -392,6 +392,14 @@ __device__ __forceinline__ void hist_pair(uint32_t* row, uint32_t lo, uint32_t h
   atomicAdd(&row[hi], 0x10000u);
 }
 
+// The same pair into a lane's private rows (bins 64 words apart; values
+// below the small histogram's bins, zeros included: bin 0 is replaced by the
+// zero masks' counts)
+__device__ __forceinline__ void hist_pair_private(uint32_t* row, uint32_t lo, uint32_t hi) {
+  atomicAdd(&row[64 * lo], 1u);
+  atomicAdd(&row[64 * hi], 0x10000u);
+}
+
 template <typename FT>
 struct Chunk8;
 
@@ -452,6 +460,20 @@ struct Chunk8<uint16_t> {
     const uint32_t nz = un(n0) | (un(n1) << 2) | (un(n2) << 4) | (un(n3) << 6);
     return ~(nz | (nz >> 15)) & 0xffu;
   }
+  // the private histogram's atomics (every value below kSmallBins) and the
+  // zero bits
+  __device__ __forceinline__ uint32_t hist_add_private(uint32_t* row0, uint32_t* row1) const {
+    hist_pair_private(row0, v.x & 0xffffu, v.x >> 16);
+    hist_pair_private(row1, v.y & 0xffffu, v.y >> 16);
+    hist_pair_private(row0, v.z & 0xffffu, v.z >> 16);
+    hist_pair_private(row1, v.w & 0xffffu, v.w >> 16);
+    const p16 one = {1, 1};
+    const uint32_t nz = un(__builtin_elementwise_min(pk(v.x), one)) |
+                        (un(__builtin_elementwise_min(pk(v.y), one)) << 2) |
+                        (un(__builtin_elementwise_min(pk(v.z), one)) << 4) |
+                        (un(__builtin_elementwise_min(pk(v.w), one)) << 6);
+    return ~(nz | (nz >> 15)) & 0xffu;
+  }
   __device__ __forceinline__ uint32_t max8() const {
     const uint32_t m = un(__builtin_elementwise_max(__builtin_elementwise_max(pk(v.x), pk(v.y)),
                                                     __builtin_elementwise_max(pk(v.z), pk(v.w))));
@@ -503,6 +525,16 @@ struct Chunk8<uint32_t> {
     for (int b = 0; b < 8; ++b) z |= (uint32_t)(get(b) == 0) << b;
     return z;
   }
+  __device__ __forceinline__ uint32_t hist_add_private(uint32_t* row0, uint32_t* row1) const {
+    hist_pair_private(row0, a.x, a.y);
+    hist_pair_private(row1, a.z, a.w);
+    hist_pair_private(row0, c.x, c.y);
+    hist_pair_private(row1, c.z, c.w);
+    uint32_t z = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) z |= (uint32_t)(get(b) == 0) << b;
+    return z;
+  }
   __device__ __forceinline__ uint32_t max8() const {
     return max(max(max(a.x, a.y), max(a.z, a.w)), max(max(c.x, c.y), max(c.z, c.w)));
   }
@@ -539,15 +571,31 @@ __device__ __forceinline__ ClampFn clamp_scan(ClampFn g) {
 // reuses the histogram once its bins are read.
 constexpr int kHistCopies = 4;
 constexpr int kHistRow = 512 + 8;
+// Shards whose values are all below kSmallBins (most of them: the residuals
+// of 8-bit content crowd into a few small values) count into a private
+// histogram per lane instead, word [pair][value][lane]: every lane's
+// counters sit in its own bank and no two lanes ever add to one address, so
+// the atomics neither collide nor conflict.
+constexpr int kSmallBins = 16;
+
+// The rank bytes' staging image (four rows of 1024 bytes) in the histogram
+// words past the list of present values (at most 255 entries of 16 bytes)
+constexpr int kRankStageOffset = 8192;
+constexpr int kRankStageRow = 1024 + 16;
 
 struct AnalyzeLds {
   union {
     uint32_t hist[2][kHistCopies][kHistRow];  // [class pair][copy][bin]
+    uint32_t priv[2][kSmallBins][64];         // [class pair][value][lane]
     uint4 list[512];  // by rank: {value, class pair 0 counts, class pair 1 counts, count}
   };
+  uint32_t small_tot[2 * kSmallBins];  // [class pair][value]: private counts summed
   uint8_t rank_of[512];  // value -> rank
   uint64_t zw[64];       // zero bits of symbols [64 j, 64 j + 64)
 };
+static_assert(kRankStageOffset >= 255 * 16 &&
+                  kRankStageOffset + 4 * kRankStageRow <= (int)sizeof(uint32_t) * 2 * kHistCopies * kHistRow,
+              "rank staging inside the histogram, past the list");
 
 // One wave64 per shard (one wave per block).  Lane l holds symbols
 // 512 c + 8 l + b (chunk c < 8, b < 8), loaded with eight 16-byte loads
@@ -609,27 +657,40 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
   uint32_t* const row1 = lds.hist[1][lane % kHistCopies];
   const uint32_t junk = (512u + ((uint32_t)lane >> 2 & 7u)) * 0x10001u;
   uint32_t lmax = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) lmax = max(lmax, ch[c].max8());
+  const uint32_t maxv = wave_max(lmax);
+  // (wave-uniform) every value of the shard has a bin of the private
+  // histogram
+  const bool small = maxv < (uint32_t)kSmallBins;
   int zeros = 0, starts = 0;
   uint32_t zmask[8];  // zero bits of each chunk's eight symbols
   uint32_t orv[8];    // OR of each chunk's eight symbols
   uint32_t prev_last = 0;  // symbol before lane 0's chunk is zero
+  auto count_chunks = [&](auto add) __attribute__((always_inline)) {
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    uint32_t zm = ch[c].hist_add(row0, row1, junk);
-    lmax = max(lmax, ch[c].max8());
-    orv[c] = ch[c].or8();
-    if (!full) zm &= (1u << clampi(count - (512 * c + 8 * lane), 0, 8)) - 1u;
-    zmask[c] = zm;
-    zeros += __popc(zm);
-    // the symbol before this lane's chunk: lane - 1's last, or for lane 0 the
-    // previous chunk's last symbol (lane 63)
-    const uint32_t up = from_prev_lane(zm >> 7);
-    const uint32_t before = lane == 0 ? prev_last : up;
-    starts += __popc(zm & ~((zm << 1) | (before & 1u)));
-    prev_last = (uint32_t)__builtin_amdgcn_readlane((int)(zm >> 7), 63);
+    for (int c = 0; c < 8; ++c) {
+      uint32_t zm = add(ch[c]);
+      orv[c] = ch[c].or8();
+      if (!full) zm &= (1u << clampi(count - (512 * c + 8 * lane), 0, 8)) - 1u;
+      zmask[c] = zm;
+      zeros += __popc(zm);
+      // the symbol before this lane's chunk: lane - 1's last, or for lane 0
+      // the previous chunk's last symbol (lane 63)
+      const uint32_t up = from_prev_lane(zm >> 7);
+      const uint32_t before = lane == 0 ? prev_last : up;
+      starts += __popc(zm & ~((zm << 1) | (before & 1u)));
+      prev_last = (uint32_t)__builtin_amdgcn_readlane((int)(zm >> 7), 63);
+    }
+  };
+  if (small) {
+    uint32_t* const p0 = &lds.priv[0][0][lane];
+    uint32_t* const p1 = &lds.priv[1][0][lane];
+    count_chunks([&](const Chunk8<FT>& k) { return k.hist_add_private(p0, p1); });
+  } else {
+    count_chunks([&](const Chunk8<FT>& k) { return k.hist_add(row0, row1, junk); });
   }
   __syncthreads();
-  const uint32_t maxv = wave_max(lmax);
   zeros = wave_sum(zeros);
   starts = wave_sum(starts);
   const bool exact = maxv < kHistLast;  // every value in a bin of its own
@@ -657,16 +718,39 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
     const uint32_t zero01 = wave_sum(zc[0] | (zc[1] << 16));
     const uint32_t zero23 = wave_sum(zc[2] | (zc[3] << 16));
     uint32_t c01[8], c23[8];
+    if (small) {
+      // row 16 p + v of the private histogram summed over the 64 lanes:
+      // lane l adds half l & 1 of row l >> 1, then the pair's halves meet;
+      // the totals of bins 8 l .. 8 l + 7 then go to lanes 0 and 1
+      const int row = lane >> 1;
+      const uint4* const src = (const uint4*)&lds.priv[row >> 4][row & 15][32 * (lane & 1)];
+      uint32_t t = 0;
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      uint32_t* acc = p == 0 ? c01 : c23;
+      for (int i = 0; i < 8; ++i) {
+        const uint4 q = src[i];
+        t += (q.x + q.y) + (q.z + q.w);
+      }
+      t += dpp_mov<kDppXor1>(0u, t);
+      if ((lane & 1) == 0) lds.small_tot[row] = t;
+      __syncthreads();
+      const int at = lane < 2 ? 8 * lane : 0;
 #pragma unroll
-      for (int cp = 0; cp < kHistCopies; ++cp) {
-        const uint4 q0 = *(const uint4*)&lds.hist[p][cp][8 * lane];
-        const uint4 q1 = *(const uint4*)&lds.hist[p][cp][8 * lane + 4];
-        const uint32_t w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+      for (int k = 0; k < 8; ++k) {
+        c01[k] = lane < 2 ? lds.small_tot[at + k] : 0u;
+        c23[k] = lane < 2 ? lds.small_tot[kSmallBins + at + k] : 0u;
+      }
+    } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) acc[k] = cp == 0 ? w[k] : acc[k] + w[k];
+      for (int p = 0; p < 2; ++p) {
+        uint32_t* acc = p == 0 ? c01 : c23;
+#pragma unroll
+        for (int cp = 0; cp < kHistCopies; ++cp) {
+          const uint4 q0 = *(const uint4*)&lds.hist[p][cp][8 * lane];
+          const uint4 q1 = *(const uint4*)&lds.hist[p][cp][8 * lane + 4];
+          const uint32_t w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+          for (int k = 0; k < 8; ++k) acc[k] = cp == 0 ? w[k] : acc[k] + w[k];
+        }
       }
     }
     if (lane == 0) {
@@ -750,12 +834,15 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
     for (int c = 0; c < 8; ++c) ch[c].launder();
     uint8_t* ranks = rank_area + (int64_t)(s / kRansShards) * kRankGroupBytes +
                      16 * 4 * (s % kRansShards);
-    // (non-temporal: the rank area, a byte per symbol, is read once, by
-    // k_rans, which on 16 4K RGB frames then takes 5 % less time)
+    // The pairs go to a staging image of the shard's four lanes in LDS (the
+    // histogram's words past the list, dead by now; rows kRankStageRow
+    // bytes apart, so the four rows' chunks fall on different banks), which
+    // then leaves as whole 16-byte chunks: four stores per lane instead of
+    // thirty-two 2-byte ones scattered over 256 partial lines.
+    uint8_t* const stage_rk = (uint8_t*)&lds.hist[0][0][0] + kRankStageOffset;
     auto put = [&](int c, int r, uint32_t pair) __attribute__((always_inline)) {
       const int p = 1022 - (128 * c + 2 * lane);  // byte of symbol b = r + 4 (lane r)
-      uint8_t* const at = ranks + 1024 * (p >> 4) + (p & 15);
-      __builtin_nontemporal_store((uint16_t)pair, (uint16_t*)(at + 16 * r));
+      *(uint16_t*)(stage_rk + kRankStageRow * r + p) = (uint16_t)pair;
     };
     if (distinct == (int)maxv + 1) {
       // every value up to the largest is present: a value's rank is the
@@ -776,6 +863,18 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
           put(c, r, rk1 | (rk0 << 8));
         }
       }
+    }
+    __syncthreads();
+    // chunk q of lane r (bytes 16 q .. 16 q + 15) at 1024 q + 16 r; four
+    // consecutive lanes write one shard's 64 contiguous bytes of a chunk row
+    // (non-temporal: the rank area, a byte per symbol, is read once, by
+    // k_rans, which on 16 4K RGB frames then takes 5 % less time)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = 64 * j + lane;
+      const int r = i & 3, q = i >> 2;
+      const q4u v = *(const q4u*)(stage_rk + kRankStageRow * r + 16 * q);
+      __builtin_nontemporal_store(v, (q4u*)(ranks + 1024 * q + 16 * r));
     }
   }
 
