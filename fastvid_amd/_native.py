@@ -107,6 +107,8 @@ def lib():
     handle.fvh_timing_read.argtypes = [p, i32, i32]
     handle.fvh_results_read.restype = ctypes.c_int
     handle.fvh_results_read.argtypes = [p, p, i32, p, cs, sz]
+    handle.fvh_results_copy.restype = ctypes.c_int
+    handle.fvh_results_copy.argtypes = [p, p, i32, p, cs, sz]
     handle.fvh_set_band_waves.restype = ctypes.c_int
     handle.fvh_set_band_waves.argtypes = [i32]
     handle.fvh_synth_plane.restype = ctypes.c_int

@@ -83,7 +83,7 @@ class GraphEncoder:
         with torch.cuda.device(self.device):
             self.job = EncodeJob(args).allocate()
             self.stream = torch.cuda.Stream(self.device)
-            self.graph = _capture(lambda: self.job.launch(stream=self.stream),
+            self.graph = _capture(lambda: self.job.launch(stream=self.stream, readback=False),
                                   self.stream) if graph else None
 
     def resident_bytes(self) -> int:
@@ -114,7 +114,7 @@ class GraphEncoder:
             with torch.cuda.stream(self.stream):
                 self.graph.replay()
         else:
-            self.job.launch(stream=self.stream)
+            self.job.launch(stream=self.stream, readback=False)
 
     def finish(self) -> list:
         self.stream.synchronize()

@@ -46,6 +46,42 @@ def _read_results(results: torch.Tensor, stream, host: torch.Tensor = None) -> n
         len(msg)), msg)
     return host.numpy().view(_RESULT_DTYPE)
 
+
+class _Readback:
+    """A job's result records copied to pinned host memory right behind its
+    kernels: ``enqueue`` (at launch) adds the copy to the job's stream and
+    records an event after it, ``wait`` (at finish) waits for that event
+    only.  The read-back of one job then waits neither for the work queued
+    after it on the stream (the next job, the next batch) nor leaves the
+    device idle while the host enqueues the copy: on 16 4K RGB frames the
+    device idled 69 us between an encode job's read-back and a decode job's
+    when each copy was enqueued at finish."""
+
+    def __init__(self, results: torch.Tensor):
+        self.results = results
+        self.count = results.numel() // _native.RESULT_BYTES
+        self.host = _pinned_results(self.count)
+        self.done = torch.cuda.Event()
+        self.pending = False
+
+    def enqueue(self, stream) -> None:
+        msg = _native.message_buffer()
+        _native.check(_native.lib().fvh_results_copy(
+            ctypes.c_void_p(self.results.data_ptr()), ctypes.c_void_p(self.host.data_ptr()),
+            self.count, ctypes.c_void_p(stream.cuda_stream), msg, len(msg)), msg)
+        self.done.record(stream)
+        self.pending = True
+
+    def wait(self, stream) -> np.ndarray:
+        """The records of the last launch (read now, synchronously, on
+        `stream` when no copy is pending: a launch captured in a graph)."""
+        if not self.pending:
+            return _read_results(self.results, stream, self.host)
+        self.done.synchronize()
+        self.pending = False
+        return self.host.numpy().view(_RESULT_DTYPE)
+
+
 _LAYOUTS = {"gray": 0, "yuv422": 1, "rgb444": 2}
 
 
@@ -187,7 +223,7 @@ class EncodeJob:
             max(self.output_bytes, 1), dtype=torch.uint8, device=self.device)
         self.results = results if results is not None else torch.empty(
             n * _native.RESULT_BYTES, dtype=torch.uint8, device=self.device)
-        self._host_results = _pinned_results(n)
+        self._readback = _Readback(self.results)
         self._views = None  # (stream lengths, their views of the output)
         return self
 
@@ -195,7 +231,10 @@ class EncodeJob:
         """Every device tensor the launch reads or writes."""
         return [self.workspace, self.output, self.results, *self._keep]
 
-    def launch(self, stream=None):
+    def launch(self, stream=None, readback: bool = True):
+        """Enqueues the encode on `stream` (the current stream by default),
+        then, with `readback`, the copy of its result records (leave it out
+        when capturing the launch in a graph; ``finish`` then reads them)."""
         if self.workspace is None:
             self.allocate()
         lib = _native.lib()
@@ -207,10 +246,12 @@ class EncodeJob:
                 ctypes.addressof(self._structs), len(self.frames), self.workspace.data_ptr(),
                 self.workspace.numel(), self.output.data_ptr(), self.output.numel(),
                 self.results.data_ptr(), ctypes.c_void_p(stream.cuda_stream), msg, len(msg)), msg)
+            if readback:
+                self._readback.enqueue(stream)
         return self
 
     def finish(self):
-        rec = _read_results(self.results, self._stream, self._host_results)
+        rec = self._readback.wait(self._stream)
         bad = np.flatnonzero(rec["status"])
         if bad.size:
             status = int(rec["status"][bad[0]])
@@ -347,7 +388,7 @@ class DecodeJob:
         self._outs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.outputs])
         # The plane views of every output are fixed once the outputs are.
         self._frames = [self.planes(i) for i in range(n)]
-        self._host_results = _pinned_results(n)
+        self._readback = _Readback(self.results)
         return self
 
     def device_buffers(self) -> list:
@@ -365,7 +406,9 @@ class DecodeJob:
                     host = staging
                 self.inputs[i].copy_(host, non_blocking=True)
 
-    def launch(self, stream=None):
+    def launch(self, stream=None, readback: bool = True):
+        """Enqueues the decode on `stream` (the current stream by default),
+        then, with `readback`, the copy of its result records."""
         if self.workspace is None:
             self.allocate()
         n = len(self.inputs)
@@ -381,6 +424,8 @@ class DecodeJob:
                 self.workspace.data_ptr(),
                 self.workspace.numel(), self.results.data_ptr(), int(self.wavefront),
                 ctypes.c_void_p(stream.cuda_stream), msg, len(msg)), msg)
+            if readback:
+                self._readback.enqueue(stream)
         return self
 
     def planes(self, i: int):
@@ -396,7 +441,7 @@ class DecodeJob:
         return tuple(planes)
 
     def finish(self):
-        rec = _read_results(self.results, self._stream, self._host_results)
+        rec = self._readback.wait(self._stream)
         bad = np.flatnonzero(rec["status"])
         if bad.size:
             raise RuntimeError(

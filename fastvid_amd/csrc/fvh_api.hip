@@ -773,6 +773,21 @@ int fvh_results_read(const fvh_result* results, fvh_result* host, int32_t count,
   return FVH_OK;
 }
 
+int fvh_results_copy(const fvh_result* results, fvh_result* host, int32_t count,
+                     fvh_stream_t stream, char* msg, size_t msg_len) {
+  if (results == nullptr || host == nullptr || count <= 0) {
+    set_msg(msg, msg_len, "results, host and count must be set");
+    return FVH_EARG;
+  }
+  const hipError_t e = hipMemcpyAsync(host, results, sizeof(fvh_result) * (size_t)count,
+                                      hipMemcpyDeviceToHost, (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_msg(msg, msg_len, hipGetErrorString(e));
+    return FVH_EHIP;
+  }
+  return FVH_OK;
+}
+
 int fvh_set_band_waves(int32_t waves_per_cu) {
   if (waves_per_cu < 0 || waves_per_cu > 64) return -1;
   return fvh::band_waves_per_cu(waves_per_cu);

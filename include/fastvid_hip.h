@@ -150,6 +150,13 @@ int fvh_decode_launch_sized(const fvh_stream_info* infos, const void* const* str
 int fvh_results_read(const fvh_result* results, fvh_result* host, int32_t count,
                      fvh_stream_t stream, char* msg, size_t msg_len);
 
+/* The same copy, only enqueued on `stream` (no wait): a job that enqueues it
+ * right behind its kernels can later wait for just that copy (an event
+ * recorded after it), so the read-back of one job does not hold back the
+ * work queued behind it on the stream (same reference counterpart). */
+int fvh_results_copy(const fvh_result* results, fvh_result* host, int32_t count,
+                     fvh_stream_t stream, char* msg, size_t msg_len);
+
 /* Per-kernel timing (no counterpart in the reference, which is timed from
  * outside with CUDA events around whole calls, scripts/evaluate.py:447-463).
  * While enabled, every kernel launch of the library is bracketed by two
