@@ -54,6 +54,12 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--frames", type=int, default=FRAMES)
     ap.add_argument("--latency-reps", type=int, default=3)
+    ap.add_argument("--no-two-streams", action="store_true",
+                    help="skip the figure of encode and decode on two streams at once (kernel "
+                         "trace runs: its overlapping kernels would enter the per-kernel times)")
+    ap.add_argument("--batch-latency-reps", type=int, default=100,
+                    help="batches timed one at a time for the batched form's latency "
+                         "percentiles (at least --steps)")
     ap.add_argument("--cpu-trials", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--streams", type=int, default=1,
@@ -239,10 +245,10 @@ def main():
     # the time from launch until every frame of the batch is done
     enc_job = sej if args.streams > 1 else ejob
     dec_job = sdj if args.streams > 1 else djob
-    # (100 batches: a p99 over 20 would be the one slowest batch, so a single
-    # stall of the host thread would set it)
+    # (100 batches by default: a p99 over 20 would be the one slowest batch,
+    # so a single stall of the host thread would set it)
     lat_benc, lat_bdec = [], []
-    for _ in range(max(args.steps, 100)):
+    for _ in range(max(args.steps, args.batch_latency_reps)):
         t = time.perf_counter()
         enc_job.launch()
         enc_job.finish()
@@ -284,16 +290,18 @@ def main():
         ejob.finish()
         djob.finish()
 
-    for _ in range(max(1, args.warmup)):
-        step_two_streams()
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        step_two_streams()
-    torch.cuda.synchronize()
-    overlap_ms = (time.perf_counter() - t) * 1e3 / args.steps
-    two_streams = {"ms_per_step": round(overlap_ms, 3),
-                   "gpps": round(luma / (overlap_ms / 1e3) / 1e9, 3)}
+    two_streams = None
+    if not args.no_two_streams:
+        for _ in range(max(1, args.warmup)):
+            step_two_streams()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            step_two_streams()
+        torch.cuda.synchronize()
+        overlap_ms = (time.perf_counter() - t) * 1e3 / args.steps
+        two_streams = {"ms_per_step": round(overlap_ms, 3),
+                       "gpps": round(luma / (overlap_ms / 1e3) / 1e9, 3)}
 
     # encode-only / decode-only throughput of the batched form
     def timed(fn, reps):
