@@ -1132,9 +1132,12 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   const q2 cb = __builtin_nontemporal_load(cp + 64 * min(1, last));
   uint64_t a0 = ca.x, a1 = ca.y, b0 = cb.x, b1 = cb.y;
   int ci = 0;
-  // byte queue: `pend` (< 8) bytes, the most recent in the low byte of qq
+  // byte queue: pbits / 8 (< 8) bytes, the most recent in the low byte of
+  // qq; `stored` bytes (a multiple of 8) already went to scratch.  (The
+  // queue's length is kept in bits: as a byte count p, the shift 64 - 8 p
+  // was compiled to a quarter-rate multiply by 56 once a round.)
   uint64_t qq = 0;
-  int pend = 0, blocks = 0;
+  uint32_t pbits = 0, stored = 0;
   // byte blocks go through a buffer resource over the group's 16 scratch
   // areas; rounds with nothing to store use an offset past it, which the
   // hardware drops (a shared sink line would take every wave's stores)
@@ -1185,22 +1188,22 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
         const uint32_t xn = (xs + (e[t].y & 0xffffu)) + __umul24(q, e[t].y >> 16);
         x = live ? xn : x;
       }
-      const int total = pend + (int)(wbits >> 3);
-      const bool full = total >= 8;
-      const uint32_t d = (uint32_t)(8 * (total - 8)) & 63u;
-      // the oldest eight bytes: the queue's pend bytes above the round's
-      const uint64_t word = (qq << ((64u - 8u * (uint32_t)pend) & 63u)) | (wd >> d);
+      const uint32_t tb = pbits + wbits;  // < 128
+      const bool full = tb >= 64u;
+      const uint32_t d = (tb - 64u) & 63u;  // the round's bits left after the word
+      // the oldest eight bytes: the queue's bytes above the round's
+      const uint64_t word = (qq << ((64u - pbits) & 63u)) | (wd >> d);
       const uint64_t keep = wd & ((1ull << d) - 1ull);
       const uint64_t merged = (qq << (wbits & 63u)) | wd;
       qq = full ? keep : merged;
-      pend = full ? total - 8 : total;
-      const bool fits = 8 * (blocks + 1) <= kLaneBytes;
-      const uint32_t off = full && fits && act ? lane_end_off - 8u * (uint32_t)(blocks + 1)
-                                               : 0x80000000u;
+      pbits = full ? d : tb;
+      const uint32_t next = stored + 8u;
+      const bool fits = next <= (uint32_t)kLaneBytes;
+      const uint32_t off = full && fits && act ? lane_end_off - next : 0x80000000u;
       using u32x2 = uint32_t __attribute__((ext_vector_type(2)));
       __builtin_amdgcn_raw_buffer_store_b64(u32x2{(uint32_t)word, (uint32_t)(word >> 32)}, lanes_r,
                                             off, 0, 0);
-      blocks += full ? 1 : 0;
+      stored = full ? next : stored;
     }
     a0 = b0;
     a1 = b1;
@@ -1213,10 +1216,11 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   for (; j0 < wave_j; j0 += 16) block(j0, true);
   if (act) {
     // the last pend (< 8) bytes: byte i of the queue (i = 0 most recent) is
-    // emitted byte 8 blocks + pend - 1 - i, stored at lane_end - 1 - that
-    nbytes = 8 * blocks + pend;
+    // emitted byte stored + pend - 1 - i, stored at lane_end - 1 - that
+    const int pend = (int)(pbits >> 3);
+    nbytes = (int)stored + pend;
     for (int i = 0; i < pend; ++i) {
-      const int e_i = 8 * blocks + pend - 1 - i;
+      const int e_i = (int)stored + pend - 1 - i;
       if (e_i < kLaneBytes) lane_end[-1 - e_i] = (uint8_t)(qq >> (8 * i));
     }
     if (nbytes > kLaneBytes) set_status(st, kStatusLaneOverflow);
