@@ -1177,9 +1177,14 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
         const uint32_t lim = e[t].x;
         uint32_t sh = (x >> 8) >= lim ? 16u : x >= lim ? 8u : 0u;
         if (!live) sh = 0;
-        // the bytes in emission order, low then high: byte-swapped low half
-        const uint32_t pv = __builtin_amdgcn_perm(0u, x, 0x0c0c0001u);
-        wd = (wd << sh) | (pv >> (16u - sh));
+        // the bytes in emission order, low then high, at the top of a dword
+        // (x's byte 0 in byte 3, byte 1 in byte 2): the word's new low half
+        // is then the high half of {low half, that} << sh, and its new high
+        // half the high half of the word << sh
+        const uint32_t pv = __builtin_amdgcn_perm(0u, x, 0x00010c0cu);
+        const uint32_t lo = (uint32_t)(((((uint64_t)(uint32_t)wd) << 32) | pv) << sh >> 32);
+        const uint32_t hi = (uint32_t)((wd << sh) >> 32);
+        wd = ((uint64_t)hi << 32) | lo;
         wbits += sh;
         const uint32_t xs = x >> sh;
         const double rc = __longlong_as_double((int64_t)(((uint64_t)e[t].w << 32) | e[t].z));
