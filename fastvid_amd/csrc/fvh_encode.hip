@@ -563,13 +563,16 @@ __device__ __forceinline__ ClampFn clamp_scan(ClampFn g) {
 }
 
 // Per-wave LDS of k_analyze.  Histogram rows are kHistRow words apart, so
-// the four copies of a value fall eight banks apart; lane l counts into
-// copy l % kHistCopies, which quarters the atomics that meet on one address
-// (the residuals of a shard crowd into a few small values; two copies took
-// 17% longer, eight left too few waves per compute unit).  Words 512..519
-// of a row are the zeros' junk slots.  The list of present values by rank
-// reuses the histogram once its bins are read.
-constexpr int kHistCopies = 4;
+// the copies of a value fall on different banks; lane l counts into copy
+// l % kHistCopies, which halves the atomics that meet on one address (the
+// residuals of a shard crowd into a few small values).  Words 512..519 of a
+// row are the zeros' junk slots.  The list of present values by rank reuses
+// the histogram once its bins are read.  The kernel is bound by the latency
+// of each wave's phases, so the copies trade atomic collisions against waves
+// per compute unit: on 16 4K RGB frames (most shards on the private
+// histogram below) 4 copies and 9 waves took 319 us, 3 and 11 waves 278 us,
+// 2 and 17 waves 250 us.
+constexpr int kHistCopies = 2;
 constexpr int kHistRow = 512 + 8;
 // Shards whose values are all below kSmallBins (most of them: the residuals
 // of 8-bit content crowd into a few small values) count into a private
@@ -580,7 +583,7 @@ constexpr int kSmallBins = 16;
 
 // The rank bytes' staging image (four rows of 1024 bytes) in the histogram
 // words past the list of present values (at most 255 entries of 16 bytes)
-constexpr int kRankStageOffset = 8192;
+constexpr int kRankStageOffset = 4096;
 constexpr int kRankStageRow = 1024 + 16;
 
 struct AnalyzeLds {
