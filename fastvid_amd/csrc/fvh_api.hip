@@ -645,10 +645,12 @@ int decode_launch(const fvh_stream_info* infos, const void* const* streams,
   int64_t tiles = 0, shards = 0, samples = 0, groups = 0;
   int band_ring = 16;
   bool any_band = false, any_nonband = false, any_med = false, any_grad = false;
+  bool any_deep = false;
   for (int32_t i = 0; i < count; ++i) {
     const fvh_stream_info& h = infos[i];
     FrameDesc& f = frames[i];
     const Geometry g = fill_geometry(f, h.width, h.height, h.tile_width, h.tile_height, h.layout);
+    any_deep |= h.bit_depth > 8;
     f.stream = (uint8_t*)streams[i];
     // with device sizes, k_dheader sets the size once the header checks out
     f.stream_capacity = fed_sizes != nullptr ? 0 : h.stream_size;
@@ -718,6 +720,7 @@ int decode_launch(const fvh_stream_info* infos, const void* const* streams,
   b.any_nonband = any_nonband;
   b.any_med = any_med;
   b.any_grad = any_grad;
+  b.any_deep = any_deep;
   b.fed_sizes = fed_sizes;
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();

@@ -677,6 +677,27 @@ struct ByteQueue {
   __device__ __forceinline__ bool drained() const { return qn == 0 && left == 0; }
 };
 
+// Files order-0 list position `item` (a valid table of nsym symbols) for the
+// bucket decoder instance sized for its alphabet: one atomic per wave and
+// tier, over the lanes of the wave that reach this call.
+__device__ __forceinline__ void file_o0_tier(int item, int nsym, int32_t* __restrict__ list_count,
+                                             int32_t* __restrict__ tier_lists, int tier_stride) {
+  const int tier = nsym <= o0_tier_cap(0) ? 0 : nsym <= o0_tier_cap(1) ? 1 : 2;
+  const int lane = lane_id();
+#pragma unroll
+  for (int t = 0; t < kO0Tiers; ++t) {
+    const uint64_t m = ballot(tier == t);
+    if (m == 0) continue;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&list_count[2 + t], popc64(m));
+    base = shfl(base, leader);
+    if (tier == t) {
+      tier_lists[t * tier_stride + base + popc64(m & ((1ull << lane) - 1))] = item;
+    }
+  }
+}
+
 // Order-0 table parse (decode.cu:441-509), one thread per order-0 shard: the
 // sparse table (table log, symbol count, value deltas and frequencies as
 // canonical LEB128), three lane lengths and four initial states, with the
@@ -801,22 +822,9 @@ __global__ __launch_bounds__(256) void k_o0_parse(const FrameDesc* __restrict__ 
   }
   tab->ent[nsym] = 1u << kRansTableLog;
   tab->nsym = (int32_t)nsym;
-  // file the shard for the decoder instance sized for its alphabet: one
-  // atomic per wave and tier (every valid shard of the wave is here)
-  const int tier = nsym <= (uint32_t)o0_tier_cap(0) ? 0 : nsym <= (uint32_t)o0_tier_cap(1) ? 1 : 2;
-  const int lane = lane_id();
-#pragma unroll
-  for (int t = 0; t < kO0Tiers; ++t) {
-    const uint64_t m = ballot(tier == t);
-    if (m == 0) continue;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&list_count[2 + t], popc64(m));
-    base = shfl(base, leader);
-    if (tier == t) {
-      tier_lists[t * tier_stride + base + popc64(m & ((1ull << lane) - 1))] = item;
-    }
-  }
+  // file the shard for the decoder instance sized for its alphabet (every
+  // valid shard of the wave is here)
+  file_o0_tier(item, (int)nsym, list_count, tier_lists, tier_stride);
 }
 
 // Latency form of the table parse, one wave per order-0 shard: the
@@ -968,6 +976,21 @@ __global__ __launch_bounds__(64) void k_o0_parse_wave(const FrameDesc* __restric
       lane_id() == 0) {
     set_status(st, 5);
   }
+}
+
+// Files the order-0 shards the wave parse accepted (nsym > 0) for the bucket
+// decoder instances, one thread per shard: the filing of k_o0_parse, for
+// batches whose tables the wave parse reads.  (Filing from the wave parse
+// itself, one atomic per shard on three counters, serialised at the L2: on
+// 16 4K RGB frames the parse then took 1.1 ms.)
+__global__ __launch_bounds__(256) void k_o0_file(int32_t* __restrict__ list_count,
+                                                 int32_t* __restrict__ tier_lists, int tier_stride,
+                                                 const O0Tab* __restrict__ tabs) {
+  const int item = blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= list_count[0]) return;
+  const int nsym = tabs[item].nsym;
+  if (nsym <= 0) return;
+  file_o0_tier(item, nsym, list_count, tier_lists, tier_stride);
 }
 
 // Order-0 rANS lanes (decode.cu:511-549): sixteen shards per wave, one quad
@@ -1805,11 +1828,24 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
   static const int direct_per_cu = blocks_per_cu((const void*)k_dec_o0_direct<FT>, 0);
   const bool latency_form = b.shard_total <= kO0DirectShards * direct_per_cu * device_cus();
   int32_t* const tier_lists = b.lists + 2 * b.shard_total;
-  if (latency_form) {
-    FVH_TIMED("k_o0_parse_wave", stream);
-    hipLaunchKernelGGL(k_o0_parse_wave, dim3(max(b.shard_total, 1)), dim3(64), 0, stream,
-                       b.frames, b.states, b.dshards, b.lists, b.list_count, b.o0tabs,
-                       (char*)b.folded);
+  // Batches holding a frame of more than 8 bits (whose shards' tables run to
+  // hundreds of symbols) parse with the wave form too, then file the shards
+  // for the bucket decoder; a thread's serial walk of a long table took 147
+  // us on four 4K gray 10-bit lossless frames, the wave form 10.  Short
+  // tables, as in 8-bit frames, parse faster a thread each (16 4K RGB
+  // frames: 47 us, the wave form 51).
+  if (latency_form || b.any_deep) {
+    {
+      FVH_TIMED("k_o0_parse_wave", stream);
+      hipLaunchKernelGGL(k_o0_parse_wave, dim3(max(b.shard_total, 1)), dim3(64), 0, stream,
+                         b.frames, b.states, b.dshards, b.lists, b.list_count, b.o0tabs,
+                         (char*)b.folded);
+    }
+    if (!latency_form) {
+      FVH_TIMED("k_o0_file", stream);
+      hipLaunchKernelGGL(k_o0_file, dim3((b.shard_total + 255) / 256), dim3(256), 0, stream,
+                         b.list_count, tier_lists, b.shard_total, b.o0tabs);
+    }
   } else {
     FVH_TIMED("k_o0_parse", stream);
     hipLaunchKernelGGL(k_o0_parse, dim3((b.shard_total + 255) / 256), dim3(256), 0, stream,

@@ -76,6 +76,7 @@ struct DecodeBuffers {
   int any_nonband;
   int any_med;
   int any_grad;
+  int any_deep;  // a frame of more than 8 bits
 };
 
 // Compute units of the current device (cached per device).
