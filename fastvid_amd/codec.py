@@ -47,6 +47,21 @@ def _read_results(results: torch.Tensor, stream, host: torch.Tensor = None) -> n
     return host.numpy().view(_RESULT_DTYPE)
 
 
+# Idle events by device, reused across jobs: a job made per call (the plain
+# API) would otherwise create its event on the first record, a HIP call that
+# costs more than the copy it marks.
+_EVENTS: dict = {}
+
+
+def _take_event(device) -> torch.cuda.Event:
+    pool = _EVENTS.setdefault(device, [])
+    return pool.pop() if pool else torch.cuda.Event()
+
+
+def _give_event(device, event: torch.cuda.Event) -> None:
+    _EVENTS.setdefault(device, []).append(event)
+
+
 class _Readback:
     """A job's result records copied to pinned host memory right behind its
     kernels: ``enqueue`` (at launch) adds the copy to the job's stream and
@@ -61,24 +76,25 @@ class _Readback:
         self.results = results
         self.count = results.numel() // _native.RESULT_BYTES
         self.host = _pinned_results(self.count)
-        self.done = torch.cuda.Event()
-        self.pending = False
+        self.done = None  # the event recorded after the pending copy
 
     def enqueue(self, stream) -> None:
         msg = _native.message_buffer()
         _native.check(_native.lib().fvh_results_copy(
             ctypes.c_void_p(self.results.data_ptr()), ctypes.c_void_p(self.host.data_ptr()),
             self.count, ctypes.c_void_p(stream.cuda_stream), msg, len(msg)), msg)
+        if self.done is None:
+            self.done = _take_event(self.results.device)
         self.done.record(stream)
-        self.pending = True
 
     def wait(self, stream) -> np.ndarray:
         """The records of the last launch (read now, synchronously, on
         `stream` when no copy is pending: a launch captured in a graph)."""
-        if not self.pending:
+        if self.done is None:
             return _read_results(self.results, stream, self.host)
         self.done.synchronize()
-        self.pending = False
+        _give_event(self.results.device, self.done)
+        self.done = None
         return self.host.numpy().view(_RESULT_DTYPE)
 
 
@@ -303,16 +319,23 @@ def encode(planes, *, layout=None, bit_depth=10, quality=90, frame_rate=(24, 1),
 
 # ------------------------------------------------------------------ decode
 
-def _read_header(encoded: torch.Tensor) -> _native.StreamInfo:
+def _check_stream(encoded: torch.Tensor) -> None:
     if encoded.dtype != torch.uint8:
         raise RuntimeError("encoded must have dtype uint8")
     if encoded.dim() != 1:
         raise RuntimeError("encoded must be one-dimensional")
     if not torch.cuda.is_available():
         raise RuntimeError("CUDA is required")
+
+
+def _read_header(encoded: torch.Tensor, head: torch.Tensor = None) -> _native.StreamInfo:
+    """The stream's header fields, checked as the reference checks them;
+    `head` is its first (up to) 32 bytes in host memory when already read."""
+    _check_stream(encoded)
     size = encoded.numel()
-    head = encoded[: min(size, 32)]
-    head = head.cpu() if head.is_cuda else head
+    if head is None:
+        head = encoded[: min(size, 32)]
+        head = head.cpu() if head.is_cuda else head
     raw = bytes(head.contiguous().numpy().tobytes()) + b"\x00" * max(0, 32 - size)
     info = _native.StreamInfo()
     msg = _native.message_buffer()
@@ -320,6 +343,24 @@ def _read_header(encoded: torch.Tensor) -> _native.StreamInfo:
     _native.check(_native.lib().fvh_decode_header(buf, size, ctypes.byref(info), msg, len(msg)),
                   msg)
     return info
+
+
+def _device_heads(streams: Sequence[torch.Tensor]) -> dict:
+    """{index: host copy of the first (up to) 32 bytes} of the device streams
+    of a batch, read with one gather and one device-to-host copy instead of a
+    synchronising copy per stream (24 4K streams: 0.6 ms).  Empty when the
+    streams are not all uint8 vectors on one device: the per-stream reads
+    then raise the reference's messages."""
+    on_device = [i for i, s in enumerate(streams) if isinstance(s, torch.Tensor) and s.is_cuda]
+    if len(on_device) < 2:
+        return {}
+    first = streams[on_device[0]]
+    if any(streams[i].dtype != torch.uint8 or streams[i].dim() != 1 or
+           streams[i].device != first.device for i in on_device):
+        return {}
+    sizes = [min(streams[i].numel(), 32) for i in on_device]
+    heads = torch.cat([streams[i][:n] for i, n in zip(on_device, sizes)]).cpu()
+    return dict(zip(on_device, heads.split(sizes)))
 
 
 class DecodeJob:
@@ -341,8 +382,9 @@ class DecodeJob:
         self.inputs = []
         self._uploads = []  # (index, host tensor) of streams uploaded at launch
         device = None
+        heads = _device_heads(streams)
         for i, encoded in enumerate(streams):
-            info = _read_header(encoded)
+            info = _read_header(encoded, heads.get(i))
             if encoded.is_cuda:
                 dev_stream = encoded.contiguous()
                 dev = dev_stream.device

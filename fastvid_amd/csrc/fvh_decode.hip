@@ -979,18 +979,45 @@ __global__ __launch_bounds__(64) void k_o0_parse_wave(const FrameDesc* __restric
 }
 
 // Files the order-0 shards the wave parse accepted (nsym > 0) for the bucket
-// decoder instances, one thread per shard: the filing of k_o0_parse, for
-// batches whose tables the wave parse reads.  (Filing from the wave parse
-// itself, one atomic per shard on three counters, serialised at the L2: on
-// 16 4K RGB frames the parse then took 1.1 ms.)
-__global__ __launch_bounds__(256) void k_o0_file(int32_t* __restrict__ list_count,
-                                                 int32_t* __restrict__ tier_lists, int tier_stride,
-                                                 const O0Tab* __restrict__ tabs) {
-  const int item = blockIdx.x * blockDim.x + threadIdx.x;
-  if (item >= list_count[0]) return;
-  const int nsym = tabs[item].nsym;
-  if (nsym <= 0) return;
-  file_o0_tier(item, nsym, list_count, tier_lists, tier_stride);
+// decoder instances, a thread per shard and one atomic per block of 1024 and
+// tier (the per-wave counts summed in LDS).  Atomics on one address
+// serialise at the L2, about 8 ns each: filing from the wave parse itself
+// (one atomic per shard) took 1.1 ms on 16 4K RGB frames, and one atomic
+// per wave and tier 55 us on 24 4K RGB 10-bit frames.
+constexpr int kFileThreads = 1024;
+
+__global__ __launch_bounds__(kFileThreads) void k_o0_file(int32_t* __restrict__ list_count,
+                                                          int32_t* __restrict__ tier_lists,
+                                                          int tier_stride,
+                                                          const O0Tab* __restrict__ tabs) {
+  constexpr int kWaves = kFileThreads / 64;
+  __shared__ int wave_count[kO0Tiers][kWaves];
+  __shared__ int block_base[kO0Tiers];
+  const int item = blockIdx.x * kFileThreads + (int)threadIdx.x;
+  const int w = (int)threadIdx.x >> 6;
+  const int lane = lane_id();
+  const int nsym = item < list_count[0] ? tabs[item].nsym : 0;
+  const int tier = nsym <= 0 ? -1 : nsym <= o0_tier_cap(0) ? 0 : nsym <= o0_tier_cap(1) ? 1 : 2;
+  uint64_t mine = 0;  // the lanes of this wave filed in this lane's tier
+#pragma unroll
+  for (int t = 0; t < kO0Tiers; ++t) {
+    const uint64_t m = ballot(tier == t);
+    if (tier == t) mine = m;
+    if (lane == 0) wave_count[t][w] = popc64(m);
+  }
+  __syncthreads();
+  if (threadIdx.x < kO0Tiers) {
+    const int t = (int)threadIdx.x;
+    int total = 0;
+    for (int i = 0; i < kWaves; ++i) total += wave_count[t][i];
+    block_base[t] = total > 0 ? atomicAdd(&list_count[2 + t], total) : 0;
+  }
+  __syncthreads();
+  if (tier >= 0) {
+    int at = block_base[tier] + popc64(mine & ((1ull << lane) - 1));
+    for (int i = 0; i < w; ++i) at += wave_count[tier][i];
+    tier_lists[tier * tier_stride + at] = item;
+  }
 }
 
 // Order-0 rANS lanes (decode.cu:511-549): sixteen shards per wave, one quad
@@ -1843,8 +1870,9 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
     }
     if (!latency_form) {
       FVH_TIMED("k_o0_file", stream);
-      hipLaunchKernelGGL(k_o0_file, dim3((b.shard_total + 255) / 256), dim3(256), 0, stream,
-                         b.list_count, tier_lists, b.shard_total, b.o0tabs);
+      hipLaunchKernelGGL(k_o0_file, dim3((b.shard_total + kFileThreads - 1) / kFileThreads),
+                         dim3(kFileThreads), 0, stream, b.list_count, tier_lists, b.shard_total,
+                         b.o0tabs);
     }
   } else {
     FVH_TIMED("k_o0_parse", stream);
