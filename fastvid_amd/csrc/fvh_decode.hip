@@ -1047,9 +1047,9 @@ __global__ __launch_bounds__(kFileThreads) void k_o0_file(int32_t* __restrict__ 
 // round needs at most eight, taken big-endian from a window of three chunks
 // of the lane's bytes (see `iteration`).  While every lane of the wave has
 // four symbols left the rounds run unpredicated; a tail loop finishes lanes of
-// uneven length.  Decoded values go to a
-// 128-byte stage per shard in LDS (O0Block) and leave as two 16-byte stores
-// per lane every four rounds (16-bit output; every two for 32-bit).  Errors (decode.cu:536-548): a lane that needs a byte past
+// uneven length.  Decoded values go to a stage per shard in LDS (O0Block)
+// and leave every two rounds as one 16-byte store per lane (16-bit output;
+// two for 32-bit).  Errors (decode.cu:536-548): a lane that needs a byte past
 // its end consumes more bytes than it has, so the final check "bytes consumed
 // == lane length and state == 2^23" covers every failure of the reference.
 constexpr int kO0Shards = 16;
@@ -1118,25 +1118,28 @@ struct O0Stage {
   static constexpr int kWords = (32 * (int)sizeof(FT) / 4) | 1;
 };
 
-// The order-0 decoder's stage holds one 128-byte block of each shard's
-// output (64 symbols of 16 bits, two iterations; 32 of 32 bits, one), so a
-// flush writes whole 128-byte lines: four lanes, two 16-byte stores each.
-// Blocks of 64 bytes left every line half-written between flushes: on 16
-// 4K RGB frames the kernel then wrote 1.03x and fetched 1.2x more bytes
-// (HBM traffic 1.73x its algorithmic bytes, now 1.54x).
-template <typename FT>
+// The order-0 decoders' stage holds a block of each shard's output, flushed
+// as four lanes' non-temporal 16-byte stores: whole 128-byte lines
+// (`kLines`: 64 symbols of 16 bits, two iterations; 32 of 32 bits, one) or
+// one iteration's 32 symbols.  The bucket decoder takes the iteration
+// blocks: 64 fewer bytes of LDS per shard for 16-bit output hold it at 12
+// waves per compute unit instead of 11, and with non-temporal stores the
+// half lines cost no extra traffic.  The direct decoder, bound by one
+// wave's chain, keeps whole lines (1080p RGB 10-bit: 147 us, 322 with
+// iteration blocks).
+template <typename FT, bool kLines>
 struct O0Block {
-  static constexpr int kPerLane = 32 / (int)sizeof(FT);  // symbols a lane stores
+  static constexpr int kPerLane = kLines ? 32 / (int)sizeof(FT) : 8;  // symbols a lane stores
   static constexpr int kSteps = kPerLane;                 // steps of a block (4 lanes)
   static constexpr int kIters = kSteps / 8;               // 8-step iterations
-  static constexpr int kWords = 33;                       // 128 bytes + 1 (odd stride)
+  static constexpr int kWords = kPerLane * (int)sizeof(FT) + 1;  // the block + 1 (odd stride)
 };
 
 template <typename FT, int kMax>
 struct O0Lds {
   uint32_t bucket[kO0Shards][O0Buckets<kMax>::kWords];
   uint2 row[kO0Shards][O0Row<kMax>::kPairs];
-  uint32_t stage[kO0Shards][O0Block<FT>::kWords];
+  uint32_t stage[kO0Shards][O0Block<FT, false>::kWords];
 };
 
 // Slot -> symbol of the bucket tables (k_dec_o0): slot -> rank by the word
@@ -1172,10 +1175,10 @@ struct BucketLookup {
 
 // The rANS lanes of one shard, one quad of lanes per shard, lane r running
 // rANS lane r (decode.cu:530-549), with the slot lookup `look`.  Decoded
-// values pass through the shard's 128-byte `stage` row and leave as whole
-// lines; lanes without a shard (item < 0 or a rejected table) decode
-// nothing of use and store to their sink slot.
-template <typename FT, typename Lookup>
+// values pass through the shard's `stage` row and leave as whole 64-byte
+// (16-bit) or 128-byte (32-bit) blocks; lanes without a shard (item < 0 or a
+// rejected table) decode nothing of use and store to their sink slot.
+template <typename FT, bool kLines, typename Lookup>
 __device__ __forceinline__ void o0_lanes(const Lookup& look, int item, int nsym,
                                          const O0Tab* tab, int r, int lane,
                                          const FrameDesc* __restrict__ frames,
@@ -1183,7 +1186,8 @@ __device__ __forceinline__ void o0_lanes(const Lookup& look, int item, int nsym,
                                          const int32_t* __restrict__ lists,
                                          FrameState* __restrict__ states,
                                          FT* __restrict__ folded, FT* const stage, char* gbase,
-                                         uint8_t* __restrict__ sink, const void* shared_line) {
+                                         uint8_t* __restrict__ sink, const void* shared_line,
+                                         bool fair = false) {
   DShard d = {};
   if (item >= 0) d = dshards[lists[item]];
   const FrameDesc& f = frames[d.frame];
@@ -1243,15 +1247,17 @@ __device__ __forceinline__ void o0_lanes(const Lookup& look, int item, int nsym,
   };
   // the block's symbols (from step s0): lane r stores kPer of them from
   // symbol kPer * r, 32 bytes
-  constexpr int kPer = O0Block<FT>::kPerLane;
+  using Blk = O0Block<FT, kLines>;
+  constexpr int kPer = Blk::kPerLane;
   // (non-temporal: the residuals are read once, by the reconstruction, and
   // streaming them past the L2 keeps the lanes' body lines there between
   // their chunk loads)
   auto flush_full = [&](int s0) __attribute__((always_inline)) {
     using q4 = uint32_t __attribute__((ext_vector_type(4)));
     FT* const dst = flush_base + (4 * s0 + kPer * r) * flush_step;
-    __builtin_nontemporal_store(((const q4*)(stage + kPer * r))[0], (q4*)dst);
-    __builtin_nontemporal_store(((const q4*)(stage + kPer * r))[1], (q4*)dst + 1);
+#pragma unroll
+    for (int i = 0; i < kPer * (int)sizeof(FT) / 16; ++i)
+      __builtin_nontemporal_store(((const q4*)(stage + kPer * r))[i], (q4*)dst + i);
   };
   auto flush = [&](int s0) __attribute__((always_inline)) {
     const int p0 = 4 * s0 + kPer * r;
@@ -1314,13 +1320,24 @@ __device__ __forceinline__ void o0_lanes(const Lookup& look, int item, int nsym,
     __builtin_amdgcn_sched_barrier(0);
   };
 
-  constexpr int kBlock = O0Block<FT>::kSteps;
+  constexpr int kBlock = Blk::kSteps;
   int s0 = 0;
   for (; s0 + kBlock <= full_steps; s0 += kBlock) {
+    // Fair progress (`fair`, a uniform argument): a wave's issue priority
+    // falls from 3 to 0 over each quarter of its 1024-step chains, so the
+    // waves sharing a SIMD, which the hardware otherwise serves oldest
+    // first, finish their groups closer together
+    if (fair && (s0 & 255) == 0) {
+      const int quarter = s0 >> 8;
+      if (quarter == 0) __builtin_amdgcn_s_setprio(3);
+      else if (quarter == 1) __builtin_amdgcn_s_setprio(2);
+      else if (quarter == 2) __builtin_amdgcn_s_setprio(1);
+      else __builtin_amdgcn_s_setprio(0);
+    }
 #pragma unroll
-    for (int h = 0; h < O0Block<FT>::kIters; ++h) {
+    for (int h = 0; h < Blk::kIters; ++h) {
       iteration(s0 + 8 * h, 8 * h, false);
-      if (h + 1 < O0Block<FT>::kIters) refill();
+      if (h + 1 < Blk::kIters) refill();
     }
     // every lane of the wave has the whole block here (full_steps >=
     // s0 + kBlock): unconditional stores, so the memory waits stay exact
@@ -1330,9 +1347,9 @@ __device__ __forceinline__ void o0_lanes(const Lookup& look, int item, int nsym,
   // (a block past a lane's last step decodes nothing: its steps are not live)
   for (; s0 < wave_steps; s0 += kBlock) {
 #pragma unroll
-    for (int h = 0; h < O0Block<FT>::kIters; ++h) {
+    for (int h = 0; h < Blk::kIters; ++h) {
       iteration(s0 + 8 * h, 8 * h, true);
-      if (h + 1 < O0Block<FT>::kIters) refill();
+      if (h + 1 < Blk::kIters) refill();
     }
     flush(s0);
     refill();
@@ -1350,7 +1367,7 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
                                                FrameState* __restrict__ states,
                                                FT* __restrict__ folded, char* gbase,
                                                uint8_t* __restrict__ sink,
-                                               int32_t* __restrict__ queue) {
+                                               int32_t* __restrict__ queue, int fair) {
   __shared__ O0Lds<FT, kMax> lds;
   const int lane = lane_id();
   const int g = lane >> 2;
@@ -1408,8 +1425,8 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
     __builtin_amdgcn_wave_barrier();
     // ---- the shard's rANS lanes
     BucketLookup<FT, kMax> look{bucket, ent};
-    o0_lanes<FT>(look, item, nsym, tab, r, lane, frames, dshards, lists, states, folded, stage,
-                 gbase, sink, tabs);
+    o0_lanes<FT, false>(look, item, nsym, tab, r, lane, frames, dshards, lists, states, folded,
+                        stage, gbase, sink, tabs, fair != 0);
     int next = 0;
     if (lane == 0) next = atomicAdd(queue, 1);
     grp = (int)gridDim.x + __builtin_amdgcn_readfirstlane(next);
@@ -1433,7 +1450,7 @@ struct O0DirectLds {
   uint32_t slot[kO0DirectShards][4096];
   uint32_t value[kO0DirectShards][256];
   uint32_t cum[kO0DirectShards][257];
-  uint32_t stage[kO0Shards][O0Block<FT>::kWords];
+  uint32_t stage[kO0Shards][O0Block<FT, true>::kWords];
 };
 
 struct DirectLookup {
@@ -1494,8 +1511,8 @@ __global__ __launch_bounds__(64) void k_dec_o0_direct(const FrameDesc* __restric
     }
     __builtin_amdgcn_wave_barrier();
     DirectLookup look{slot, val};
-    o0_lanes<FT>(look, item, nsym, tab, r, lane, frames, dshards, lists, states, folded,
-                 (FT*)lds.stage[g], gbase, sink, tabs);
+    o0_lanes<FT, true>(look, item, nsym, tab, r, lane, frames, dshards, lists, states, folded,
+                       (FT*)lds.stage[g], gbase, sink, tabs);
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -1888,13 +1905,28 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
                        (char*)b.folded, b.sink);
   } else {
     FVH_TIMED("k_dec_o0", stream);
-    // one persistent instance per alphabet tier; each drains its list
+    // One persistent instance per alphabet tier; each drains its list.  The
+    // grid is the waves the device holds at once: a wave past them starts
+    // only when a resident one leaves, at the end of the list, and then
+    // decodes the group its block index names alone (16 4K RGB 8-bit
+    // frames, 6075 groups at 11 waves per compute unit: 451 -> 427 us).
+    // Every group is 1024 steps long and each SIMD runs its waves VALU-bound,
+    // oldest first, so the groups of a round end spread out; that
+    // pipelines a last round that is only partly filled, while a last round
+    // that is nearly full (or a single round) ends sooner with the waves kept
+    // level (`fair`): 6075 groups at 12 waves (1.98 rounds) 441 -> 412 us,
+    // but at 11 waves (2.16 rounds) 427 -> 450.  The count of groups is the
+    // order-0 shards of every tier, which for 8-bit frames are this tier's.
     const int groups = (b.shard_total + kO0Shards - 1) / kO0Shards;
-    const dim3 grid(min(groups, 16 * device_cus()));
+    static const int o0 = blocks_per_cu((const void*)k_dec_o0<FT, o0_tier_cap(0)>, 0);
+    const int r0 = o0 * device_cus();
+    const dim3 grid(min(groups, r0));
+    const int part = groups % r0;
+    const int fair = groups <= r0 || part == 0 || 4 * part >= 3 * r0;
     int32_t* tiers = b.lists + 2 * b.shard_total;
     hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(0)>), grid, dim3(64), 0, stream, b.frames,
                        b.dshards, b.lists, tiers, b.list_count + 2, b.o0tabs, b.states,
-                       (FT*)b.folded, (char*)b.folded, b.sink, b.queues + 4);
+                       (FT*)b.folded, (char*)b.folded, b.sink, b.queues + 4, fair);
     // wider tables: fewer resident waves, and waves beyond them only queue
     static const int o1 = blocks_per_cu((const void*)k_dec_o0<FT, o0_tier_cap(1)>, 0);
     static const int o2 = blocks_per_cu((const void*)k_dec_o0<FT, o0_tier_cap(2) - 1>, 0);
@@ -1902,11 +1934,11 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
     hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(1)>), dim3(min((int)grid.x, r1)), dim3(64), 0,
                        stream, b.frames, b.dshards, b.lists, tiers + b.shard_total,
                        b.list_count + 3, b.o0tabs, b.states, (FT*)b.folded, (char*)b.folded,
-                       b.sink, b.queues + 5);
+                       b.sink, b.queues + 5, 0);
     hipLaunchKernelGGL((k_dec_o0<FT, o0_tier_cap(2) - 1>), dim3(min((int)grid.x, r2)), dim3(64),
                        0, stream, b.frames, b.dshards, b.lists, tiers + 2 * b.shard_total,
                        b.list_count + 4, b.o0tabs, b.states, (FT*)b.folded, (char*)b.folded,
-                       b.sink, b.queues + 6);
+                       b.sink, b.queues + 6, 0);
   }
   {
     FVH_TIMED("k_dec_rice", stream);

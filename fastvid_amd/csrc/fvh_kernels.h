@@ -85,6 +85,9 @@ int device_cus();
 // kernel's grid beyond this times device_cus() only adds waves that start
 // after others have ended, which for an empty work list is pure dispatch.
 int blocks_per_cu(const void* kernel, int dyn_lds);
+// LDS allocation unit of a gfx950 compute unit (160 KB = 128 of them)
+constexpr size_t kLdsGranule = 1280;
+constexpr size_t kLdsGranules = 128;
 
 // Bytes of the library-owned encoder workspace of fvh_encode_launch_static
 // (one per device, held until the process ends): one 3840x2160 frame of any
