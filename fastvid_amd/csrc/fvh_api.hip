@@ -172,6 +172,7 @@ struct EncodePlan {
   int64_t output_bytes = 0;
   int64_t tiles = 0, shards = 0, samples = 0;
   bool wide_folded = false, any_heuristic = false, any_gray16 = false;
+  bool narrow_folded = true;  // until a frame's residuals may exceed a byte
   bool any_lossless = false, any_lossy = false, any_wide = false;
   bool any_band = false, any_nonband = false, any_med = false, any_grad = false;
   int64_t groups = 0;
@@ -274,6 +275,11 @@ int build_encode_plan(const fvh_encode_frame* in, int32_t count, EncodePlan& pla
     plan.shards += g.shards;
     plan.samples += g.samples;
     plan.wide_folded |= fr.bit_depth == 16;
+    // a lossy frame's quantised residuals lie in [-K, K], K = ceil(max / step)
+    // (encode.cu:183-185), so zigzagged they are at most 2K; frames whose
+    // step the heuristics may change keep 16 bits
+    plan.narrow_folded &= f.heuristic == 0 && f.step0 > 1 &&
+                          2 * ((f.max_sample + f.step0 - 1) / f.step0) <= 255;
     plan.any_heuristic |= f.heuristic != 0;
     plan.any_gray16 |= f.heuristic == 2;
     plan.any_lossless |= f.step0 == 1;
@@ -380,6 +386,7 @@ int encode_launch_planned(EncodePlan& plan, int32_t frame_count, void* workspace
   b.tile_total = (int)plan.tiles;
   b.shard_total = (int)plan.shards;
   b.wide_folded = plan.wide_folded;
+  b.narrow_folded = plan.narrow_folded && !plan.wide_folded;
   b.any_heuristic = plan.any_heuristic;
   b.any_gray16 = plan.any_gray16;
   b.any_lossless = plan.any_lossless;

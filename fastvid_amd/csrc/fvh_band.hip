@@ -56,6 +56,7 @@ constexpr int kRsrcWord3 = 0x00020000;     // raw buffer, 32-bit dwords (gfx9 fa
 constexpr int kAuxNt = 2;                  // buffer instruction cache policy: non-temporal
 
 using v4u = uint32_t __attribute__((ext_vector_type(4)));
+using v2u = uint32_t __attribute__((ext_vector_type(2)));
 using i16x2 = short __attribute__((ext_vector_type(2)));
 using u16x2 = unsigned short __attribute__((ext_vector_type(2)));
 using rsrc_t = __amdgpu_buffer_rsrc_t;
@@ -140,6 +141,22 @@ struct Vec8<uint32_t> {
   }
 };
 
+// Eight 8-bit residuals (the encoder's output when every residual of the
+// batch fits a byte).  In LDS a chunk keeps the whole quad of the 16-bit
+// input chunk it replaces, so the line buffers stay in place; in memory it
+// is 8 bytes.
+template <>
+struct Vec8<uint8_t> {
+  v2u v;
+  static constexpr int kQuads = 1;
+  template <int kAux>
+  __device__ __forceinline__ void store(rsrc_t r, uint32_t off) const {
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, kAux);
+  }
+  __device__ __forceinline__ void get_q(const v4u* q) { v = *(const v2u*)q; }
+  __device__ __forceinline__ void put_q(v4u* q) const { *(v2u*)q = v; }
+};
+
 // Quantiser of the reference, q = sign(r) * floor((|r| + h) / d) with
 // h = d / 2 (encode.cu:183-185), from the non-negative numerator
 //   n = r + h + K*d - [r < 0 and d even],   K = ceil(max / d),
@@ -170,7 +187,7 @@ __device__ __forceinline__ Quant quant_setup(int step, int max_sample, int bit_d
   q.offp = h + q.K * step;
   q.lo = -q.offp;
   q.hi = max_sample - q.offp;
-  if (sizeof(FT) == 2) {
+  if (sizeof(FT) <= 2) {
     const uint32_t c = step > 1 ? 32 - __builtin_clz((uint32_t)step - 1) : 0;
     q.s = (uint32_t)bit_depth + 2 + c;
     q.m = (uint32_t)(((1ull << q.s) + (uint64_t)step - 1) / (uint64_t)step);
@@ -208,7 +225,7 @@ __device__ __forceinline__ int add_asm(int a, int b) {
 template <typename FT, bool kOdd>
 __device__ __forceinline__ int quantize(int n, const Quant& q) {
   if (!kOdd) n -= (uint32_t)n < (uint32_t)q.offp ? 1 : 0;
-  if (sizeof(FT) == 2) return (int)mad_u24((uint32_t)n, q.m, q.negks) >> q.s;
+  if (sizeof(FT) <= 2) return (int)mad_u24((uint32_t)n, q.m, q.negks) >> q.s;
   return (int)__umulhi((uint32_t)n, q.m) - q.K;
 }
 
@@ -234,7 +251,7 @@ __device__ __forceinline__ uint32_t mad_hi16(uint32_t w, uint32_t m, uint32_t c)
 template <typename FT, bool kOdd>
 __device__ __forceinline__ int quantize_pred(uint32_t w, bool hi, int p, const Quant& q) {
   const int x = (int)(hi ? w >> 16 : w & 0xffffu);
-  if constexpr (sizeof(FT) == 2) {
+  if constexpr (sizeof(FT) <= 2) {
     const uint32_t cx = hi ? mad_hi16(w, q.m, q.negks) : mad_lo16(w, q.m, q.negks);
     int base = (int)cx;
     if (!kOdd) base = p > x - q.offp ? (int)(cx - q.m) : (int)cx;
@@ -298,8 +315,8 @@ __device__ __forceinline__ void band_group(
   using OutT = typename T::Out;
   constexpr int kQi = T::kQi;
   constexpr int kQo = T::kQo;
-  constexpr uint32_t kCi = 16 * kQi;  // bytes of one input chunk
-  constexpr uint32_t kCo = 16 * kQo;
+  constexpr uint32_t kCi = 8 * (uint32_t)sizeof(InT);  // bytes of one chunk in memory
+  constexpr uint32_t kCo = 8 * (uint32_t)sizeof(OutT);
   // the decoder's samples are the call's output: non-temporal stores keep
   // them from evicting what the next kernels read (decode 16 4K RGB frames:
   // reconstruction -17 us, order-0 decode -27 us); the encoder's residuals
@@ -458,7 +475,7 @@ __device__ __forceinline__ void band_group(
         L = rec;
         UL = A;
       }
-      if constexpr (sizeof(FT) == 2) {
+      if constexpr (sizeof(FT) <= 2) {
         // zigzag of two residuals per instruction: (q << 1) ^ (q >> 15) on 16-bit halves
         uint32_t w[4];
 #pragma unroll
@@ -467,7 +484,13 @@ __device__ __forceinline__ void band_group(
           const i16x2 F = (P << (short)1) ^ (P >> (short)15);
           w[k2] = __builtin_bit_cast(uint32_t, F);
         }
-        out.v = v4u{w[0], w[1], w[2], w[3]};
+        if constexpr (sizeof(FT) == 2) {
+          out.v = v4u{w[0], w[1], w[2], w[3]};
+        } else {
+          // (every zigzag < 256 here) the halves' low bytes, in order
+          out.v = v2u{__builtin_amdgcn_perm(w[1], w[0], 0x06040200u),
+                      __builtin_amdgcn_perm(w[3], w[2], 0x06040200u)};
+        }
       } else {
 #pragma unroll
         for (int j = 0; j < 8; ++j) out.set(j, ((uint32_t)q[j] << 1) ^ (uint32_t)(q[j] >> 31));
@@ -673,6 +696,7 @@ void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* ti
   template void launch_band<E, FT>(const FrameDesc*, FrameState*, const TileDesc*,            \
                                    const int32_t*, int, int, FT*, int32_t*, bool, bool,        \
                                    hipStream_t);
+FVH_BAND_INST(true, uint8_t)
 FVH_BAND_INST(true, uint16_t)
 FVH_BAND_INST(true, uint32_t)
 FVH_BAND_INST(false, uint16_t)

@@ -677,46 +677,60 @@ struct ByteQueue {
   __device__ __forceinline__ bool drained() const { return qn == 0 && left == 0; }
 };
 
-// Files order-0 list position `item` (a valid table of nsym symbols) for the
-// bucket decoder instance sized for its alphabet: one atomic per wave and
-// tier, over the lanes of the wave that reach this call.
-__device__ __forceinline__ void file_o0_tier(int item, int nsym, int32_t* __restrict__ list_count,
-                                             int32_t* __restrict__ tier_lists, int tier_stride) {
-  const int tier = nsym <= o0_tier_cap(0) ? 0 : nsym <= o0_tier_cap(1) ? 1 : 2;
+// Files the order-0 list positions of a block's threads for the bucket
+// decoder instance sized for each alphabet (`nsym` symbols; 0 files
+// nothing): counts by wave ballot, summed per block in LDS, one atomic per
+// block and tier.  Atomics on one address serialise at the L2, about 8 ns
+// each: one per shard took 1.1 ms on 16 4K RGB frames, one per wave and
+// tier 55 us on 24 4K RGB 10-bit frames.  Every thread of the block calls it.
+template <int kThreads>
+__device__ __forceinline__ void file_o0_block(int item, int nsym, int32_t* __restrict__ list_count,
+                                              int32_t* __restrict__ tier_lists,
+                                              int tier_stride) {
+  constexpr int kWaves = kThreads / 64;
+  __shared__ int wave_count[kO0Tiers][kWaves];
+  __shared__ int block_base[kO0Tiers];
+  const int w = (int)threadIdx.x >> 6;
   const int lane = lane_id();
+  const int tier = nsym <= 0 ? -1 : nsym <= o0_tier_cap(0) ? 0 : nsym <= o0_tier_cap(1) ? 1 : 2;
+  uint64_t mine = 0;  // the lanes of this wave filed in this lane's tier
 #pragma unroll
   for (int t = 0; t < kO0Tiers; ++t) {
     const uint64_t m = ballot(tier == t);
-    if (m == 0) continue;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&list_count[2 + t], popc64(m));
-    base = shfl(base, leader);
-    if (tier == t) {
-      tier_lists[t * tier_stride + base + popc64(m & ((1ull << lane) - 1))] = item;
-    }
+    if (tier == t) mine = m;
+    if (lane == 0) wave_count[t][w] = popc64(m);
+  }
+  __syncthreads();
+  if (threadIdx.x < kO0Tiers) {
+    const int t = (int)threadIdx.x;
+    int total = 0;
+    for (int i = 0; i < kWaves; ++i) total += wave_count[t][i];
+    block_base[t] = total > 0 ? atomicAdd(&list_count[2 + t], total) : 0;
+  }
+  __syncthreads();
+  if (tier >= 0) {
+    int at = block_base[tier] + popc64(mine & ((1ull << lane) - 1));
+    for (int i = 0; i < w; ++i) at += wave_count[tier][i];
+    tier_lists[tier * tier_stride + at] = item;
   }
 }
 
-// Order-0 table parse (decode.cu:441-509), one thread per order-0 shard: the
-// sparse table (table log, symbol count, value deltas and frequencies as
-// canonical LEB128), three lane lengths and four initial states, with the
-// reference's checks; a rejected table sets status 5.
-__global__ __launch_bounds__(256) void k_o0_parse(const FrameDesc* __restrict__ frames,
-                                                  FrameState* __restrict__ states,
-                                                  const DShard* __restrict__ dshards,
-                                                  const int32_t* __restrict__ lists,
-                                                  int32_t* __restrict__ list_count,
-                                                  int32_t* __restrict__ tier_lists, int tier_stride,
-                                                  O0Tab* __restrict__ tabs, char* gbase) {
-  const int item = blockIdx.x * blockDim.x + threadIdx.x;
-  if (item >= list_count[0]) return;
+// Order-0 table parse (decode.cu:441-509) of list position `item` by one
+// thread: the sparse table (table log, symbol count, value deltas and
+// frequencies as canonical LEB128), three lane lengths and four initial
+// states, with the reference's checks; a rejected table sets status 5.
+// Returns the symbol count, 0 for a rejected table or a failed frame.
+__device__ __forceinline__ int parse_o0_thread(int item, const FrameDesc* __restrict__ frames,
+                                               FrameState* __restrict__ states,
+                                               const DShard* __restrict__ dshards,
+                                               const int32_t* __restrict__ lists,
+                                               O0Tab* __restrict__ tabs, char* gbase) {
   const DShard d = dshards[lists[item]];
   O0Tab* tab = tabs + item;
   FrameState* st = &states[d.frame];
   if (st->status != 0) {
     tab->nsym = 0;
-    return;
+    return 0;
   }
   const FrameDesc& f = frames[d.frame];
   const uint32_t max_folded = (uint32_t)f.max_sample * 2u;
@@ -818,13 +832,31 @@ __global__ __launch_bounds__(256) void k_o0_parse(const FrameDesc* __restrict__ 
   if (!valid) {
     set_status(st, 5);
     tab->nsym = 0;
-    return;
+    return 0;
   }
   tab->ent[nsym] = 1u << kRansTableLog;
   tab->nsym = (int32_t)nsym;
-  // file the shard for the decoder instance sized for its alphabet (every
-  // valid shard of the wave is here)
-  file_o0_tier(item, (int)nsym, list_count, tier_lists, tier_stride);
+  return (int)nsym;
+}
+
+// The table parse a thread per order-0 shard, each valid shard filed for the
+// decoder instance sized for its alphabet, one atomic per block and tier
+// (16 4K RGB frames: 50 us with one per wave and tier, 45 per block of 256,
+// 54 per block of 1024, whose fewer blocks leave compute units idle)
+constexpr int kParseThreads = 256;
+
+__global__ __launch_bounds__(kParseThreads) void k_o0_parse(const FrameDesc* __restrict__ frames,
+                                                            FrameState* __restrict__ states,
+                                                            const DShard* __restrict__ dshards,
+                                                            const int32_t* __restrict__ lists,
+                                                            int32_t* __restrict__ list_count,
+                                                            int32_t* __restrict__ tier_lists,
+                                                            int tier_stride,
+                                                            O0Tab* __restrict__ tabs, char* gbase) {
+  const int item = blockIdx.x * kParseThreads + (int)threadIdx.x;
+  const int nsym =
+      item < list_count[0] ? parse_o0_thread(item, frames, states, dshards, lists, tabs, gbase) : 0;
+  file_o0_block<kParseThreads>(item, nsym, list_count, tier_lists, tier_stride);
 }
 
 // Latency form of the table parse, one wave per order-0 shard: the
@@ -979,45 +1011,16 @@ __global__ __launch_bounds__(64) void k_o0_parse_wave(const FrameDesc* __restric
 }
 
 // Files the order-0 shards the wave parse accepted (nsym > 0) for the bucket
-// decoder instances, a thread per shard and one atomic per block of 1024 and
-// tier (the per-wave counts summed in LDS).  Atomics on one address
-// serialise at the L2, about 8 ns each: filing from the wave parse itself
-// (one atomic per shard) took 1.1 ms on 16 4K RGB frames, and one atomic
-// per wave and tier 55 us on 24 4K RGB 10-bit frames.
+// decoder instances, a thread per shard (one atomic per block and tier).
 constexpr int kFileThreads = 1024;
 
 __global__ __launch_bounds__(kFileThreads) void k_o0_file(int32_t* __restrict__ list_count,
                                                           int32_t* __restrict__ tier_lists,
                                                           int tier_stride,
                                                           const O0Tab* __restrict__ tabs) {
-  constexpr int kWaves = kFileThreads / 64;
-  __shared__ int wave_count[kO0Tiers][kWaves];
-  __shared__ int block_base[kO0Tiers];
   const int item = blockIdx.x * kFileThreads + (int)threadIdx.x;
-  const int w = (int)threadIdx.x >> 6;
-  const int lane = lane_id();
   const int nsym = item < list_count[0] ? tabs[item].nsym : 0;
-  const int tier = nsym <= 0 ? -1 : nsym <= o0_tier_cap(0) ? 0 : nsym <= o0_tier_cap(1) ? 1 : 2;
-  uint64_t mine = 0;  // the lanes of this wave filed in this lane's tier
-#pragma unroll
-  for (int t = 0; t < kO0Tiers; ++t) {
-    const uint64_t m = ballot(tier == t);
-    if (tier == t) mine = m;
-    if (lane == 0) wave_count[t][w] = popc64(m);
-  }
-  __syncthreads();
-  if (threadIdx.x < kO0Tiers) {
-    const int t = (int)threadIdx.x;
-    int total = 0;
-    for (int i = 0; i < kWaves; ++i) total += wave_count[t][i];
-    block_base[t] = total > 0 ? atomicAdd(&list_count[2 + t], total) : 0;
-  }
-  __syncthreads();
-  if (tier >= 0) {
-    int at = block_base[tier] + popc64(mine & ((1ull << lane) - 1));
-    for (int i = 0; i < w; ++i) at += wave_count[tier][i];
-    tier_lists[tier * tier_stride + at] = item;
-  }
+  file_o0_block<kFileThreads>(item, nsym, list_count, tier_lists, tier_stride);
 }
 
 // Order-0 rANS lanes (decode.cu:511-549): sixteen shards per wave, one quad
@@ -1893,7 +1896,8 @@ static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
     }
   } else {
     FVH_TIMED("k_o0_parse", stream);
-    hipLaunchKernelGGL(k_o0_parse, dim3((b.shard_total + 255) / 256), dim3(256), 0, stream,
+    hipLaunchKernelGGL(k_o0_parse, dim3((b.shard_total + kParseThreads - 1) / kParseThreads),
+                       dim3(kParseThreads), 0, stream,
                        b.frames, b.states, b.dshards, b.lists, b.list_count, tier_lists,
                        b.shard_total, b.o0tabs, (char*)b.folded);
   }

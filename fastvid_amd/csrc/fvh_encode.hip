@@ -485,6 +485,20 @@ struct Chunk8<uint16_t> {
   }
 };
 
+// Eight 8-bit residuals, widened on load to the 16-bit layout (four byte
+// permutes), after which every step is the 16-bit one
+template <>
+struct Chunk8<uint8_t> : Chunk8<uint16_t> {
+  __device__ __forceinline__ void load(const uint8_t* p) {
+    using u32x2 = uint32_t __attribute__((ext_vector_type(2)));
+    const u32x2 b = __builtin_nontemporal_load((const u32x2*)p);
+    v.x = __builtin_amdgcn_perm(0u, b.x, 0x0c010c00u);
+    v.y = __builtin_amdgcn_perm(0u, b.x, 0x0c030c02u);
+    v.z = __builtin_amdgcn_perm(0u, b.y, 0x0c010c00u);
+    v.w = __builtin_amdgcn_perm(0u, b.y, 0x0c030c02u);
+  }
+};
+
 template <>
 struct Chunk8<uint32_t> {
   q4u a, c;
@@ -1871,6 +1885,7 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
 void launch_encode(const EncodeBuffers& b, hipStream_t stream) {
   launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.group_first, b.tile_total, stream);
   if (b.wide_folded) encode_all<uint32_t>(b, stream);
+  else if (b.narrow_folded) encode_all<uint8_t>(b, stream);
   else encode_all<uint16_t>(b, stream);
 }
 

@@ -36,6 +36,7 @@ struct EncodeBuffers {
   int tile_total;
   int shard_total;
   int wide_folded;       // residuals stored as uint32 (16-bit frames present)
+  int narrow_folded;     // residuals stored as uint8 (every one fits a byte)
   int any_heuristic;     // gray10 / gray16 lossy frames present
   int any_gray16;
   int any_lossless;
