@@ -31,20 +31,31 @@ sys.path.insert(0, str(ROOT))
 WIDTH, HEIGHT, FRAMES, QUALITY, TILE = 3840, 2160, 16, 90, (256, 128)
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = "pmc_traffic_4k_rgb8_q90.md"
-# library timing name -> rocprofv3 short-name prefix of this workload (16-bit
-# residuals); a timed launch that issues one kernel per alphabet tier (k_rans,
-# k_dec_o0) sums the tier instances' traffic
-PROFILE_NAMES = {
-    "k_band_enc": "k_band<true,true,u16>",
-    "k_band_dec": "k_band<false,true,u16>",
-    "k_analyze": "k_analyze<u16>",
-    "k_rans": "k_rans<u16,",
-    "k_emit": "k_emit<u16>",
-    "k_emit_o0": "k_emit_o0<u16>",
-    "k_dec_o0": "k_dec_o0<u16,",
-    "k_dec_other": "k_dec_other<u16>",
-    "k_o0_parse": "k_o0_parse",
-}
+
+
+def profile_names(enc_res):
+    """Library timing name -> rocprofv3 short-name prefix of this workload: the
+    encoder's residual type is `enc_res` ("u8" or "u16"), the decoder's u16; a
+    timed launch that issues one kernel per alphabet tier (k_rans, k_dec_o0)
+    sums the tier instances' traffic."""
+    return {
+        "k_band_enc": f"k_band<true,true,{enc_res}>",
+        "k_band_dec": "k_band<false,true,u16>",
+        "k_analyze": f"k_analyze<{enc_res}>",
+        "k_rans": f"k_rans<{enc_res},",
+        "k_emit": f"k_emit<{enc_res}>",
+        "k_emit_o0": f"k_emit_o0<{enc_res}>",
+        "k_dec_o0": "k_dec_o0<u16,",
+        "k_dec_other": "k_dec_other<u16>",
+        "k_o0_parse": "k_o0_parse",
+    }
+
+
+def encoder_residual_bytes(step, max_sample):
+    """Bytes per residual the encoder stores: one when every zigzagged residual
+    of the lossy quantiser, at most 2 ceil(max / step), fits a byte (the
+    library's rule), else two."""
+    return 1 if step > 1 and 2 * ((max_sample + step - 1) // step) <= 255 else 2
 
 
 def parse_args():
@@ -108,17 +119,19 @@ def mode_split(streams):
     return o0_samples, o0_bytes, other_samples, other_bytes
 
 
-def algorithmic_bytes(samples, stream_bytes, split):
-    """Per-kernel algorithmic bytes per step: residuals at their minimal width
-    (u16 for bit depth <= 12), uint16 samples, stream bytes."""
+def algorithmic_bytes(samples, stream_bytes, split, enc_res_bytes=2):
+    """Per-kernel algorithmic bytes per step: the encoder's residuals at the
+    width it stores them (`enc_res_bytes`), the decoder's at 16 bits, uint16
+    samples, stream bytes."""
     s, b = samples, stream_bytes
     o0_s, o0_b, ot_s, ot_b = split
+    r = enc_res_bytes
     return {
-        "k_band_enc": 2 * s + 2 * s,        # samples in, residuals out
-        "k_wavefront_enc": 2 * s + 2 * s,
-        "k_analyze": 2 * s,                 # residuals in
+        "k_band_enc": 2 * s + r * s,        # samples in, residuals out
+        "k_wavefront_enc": 2 * s + r * s,
+        "k_analyze": r * s,                 # residuals in
         "k_rans": s + b,                    # rank bytes in (one per sample), lane bytes out
-        "k_emit": 2 * ot_s + 2 * ot_b,      # other records: residuals in, records out
+        "k_emit": r * ot_s + 2 * ot_b,      # other records: residuals in, records out
         "k_emit_o0": 2 * o0_b,              # order-0 records: scratch pieces in, records out
         "k_dec_o0": o0_b + 2 * o0_s,        # order-0 bodies in, residuals out
         "k_dec_other": ot_b + 2 * ot_s,     # other bodies in, residuals out
@@ -138,7 +151,7 @@ def main():
     import torch
     from fastvid_amd import _native
     from fastvid_amd.codec import DecodeJob, EncodeJob, frame_args, encode as enc1, decode as dec1
-    from fastvid_amd.codec import side_streams, split_decode_job, split_encode_job
+    from fastvid_amd.codec import side_streams, split_decode_job, split_encode_job, _read_header
 
     _native.lib()
     device = torch.device("cuda", 0)
@@ -353,7 +366,10 @@ def main():
     # ---- roofline of the dominant kernel (HIP events on the launch stream)
     samples = 3 * WIDTH * HEIGHT * args.frames
     split = mode_split(streams)
-    alg = algorithmic_bytes(samples, stream_bytes, split)
+    head = _read_header(streams[0])
+    enc_res_bytes = encoder_residual_bytes(int(head.step), int(head.max_sample))
+    names = profile_names("u8" if enc_res_bytes == 1 else "u16")
+    alg = algorithmic_bytes(samples, stream_bytes, split, enc_res_bytes)
     stages = {}
     for name, (launches, total_ms) in kernels.items():
         per = total_ms / max(launches, 1)
@@ -368,7 +384,7 @@ def main():
     measured = read_traffic(args.traffic)
 
     def traffic_of(name):
-        prefix = PROFILE_NAMES.get(name, name)
+        prefix = names.get(name, name)
         hits = [v for k, v in measured.items() if k == prefix or (prefix.endswith(",") and k.startswith(prefix))]
         return int(sum(f + w for f, w in hits)) if hits else None
 

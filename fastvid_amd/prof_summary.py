@@ -32,7 +32,8 @@ def short_name(name: str) -> str:
     """'void fvh::k_band<true, true, unsigned short>(...)' -> 'k_band<true,true,u16>'."""
     base = name.split("(")[0]
     base = base.replace("void ", "").replace("fvh::", "").strip()
-    base = base.replace("unsigned short", "u16").replace("unsigned int", "u32").replace(" ", "")
+    base = (base.replace("unsigned short", "u16").replace("unsigned int", "u32")
+            .replace("unsigned char", "u8").replace(" ", ""))
     return base
 
 
