@@ -146,6 +146,36 @@ def test_stream_parity_with_oracle(fc, oracle, case):
         assert np.array_equal(_host(a), b)
 
 
+@pytest.mark.parametrize("quality,step", ((95, 2), (93, 3), (80, 5)))
+@pytest.mark.parametrize("width,height,tile", ((256, 96, (128, 64)), (203, 77, (64, 32))))
+def test_extreme_residuals_at_byte_boundary_match_oracle(fc, oracle, quality, step, width,
+                                                         height, tile):
+    """8-bit RGB alternating 0 and 255 in both directions, so residuals reach
+    +-255.  At step 2 a residual quantises to +-128, whose zigzag (256) does
+    not fit a byte: the encoder must keep 16-bit residuals there and may store
+    bytes from step 3 (largest zigzag 2 * ceil(255 / 3) = 170).  Band
+    (multiples of 8) and non-band geometries, each against the oracle."""
+    yy, xx = np.mgrid[0:height, 0:width]
+    rng = np.random.default_rng(width + quality)
+    planes = []
+    for p in range(3):
+        base = (((yy + xx + p) & 1) * 255).astype(np.int64)
+        flip = rng.random((height, width)) < 0.1
+        planes.append(np.where(flip, 255 - base, base).astype(np.uint16))
+    source = tuple(planes)
+    assert oracle.quantization_step(2, 8, quality) == step
+    stream = fc.encode(_cuda(source), layout="rgb444", bit_depth=8, quality=quality,
+                       tile_size=tile)
+    expected, folded, _ = oracle.encode(source, layout="rgb444", bit_depth=8, quality=quality,
+                                        tile_size=tile, intermediates=True)
+    # the content reaches the quantiser's bound, 2 * ceil(255 / step)
+    assert int(folded.max()) == 2 * ((255 + step - 1) // step)
+    assert _bytes(stream) == expected
+    decoded = fc.decode(stream)
+    for a, b in zip(decoded, oracle.decode(expected)):
+        assert np.array_equal(_host(a), b)
+
+
 @pytest.mark.parametrize("layout,width,height,tile", [
     ("gray", 160, 120, (64, 64)),
     ("rgb444", 100, 70, (256, 128)),
