@@ -41,8 +41,15 @@ constexpr int kScratchTable = 0;        // 255 x {freq | cum << 13, 0} by rank
 constexpr int kScratchLensStates = 20;  // 28 bytes of the order-0 record
 constexpr int kScratchHead = 2045;      // 3 record header bytes, just before the sparse table
 constexpr int kScratchSparse = 2048;    // sparse table bytes (<= 1024)
-constexpr int kScratchLanes = 3072;     // 4 lanes x kLaneBytes, filled from the end
+constexpr int kScratchLanes = 3072;     // 4 lanes x kLaneBytes, interleaved by 8-byte slot
 constexpr int kLaneBytes = 1536;        // proven bound for 1024 symbols
+// The lane area is kLaneSlots slots of 32 bytes, lane r at byte 8 r of each.
+// A lane's n bytes, in stream order, are bytes 1536 - n .. 1535 of its
+// virtual run, virtual byte v at slot v / 8, byte v % 8: the rANS encoder
+// fills slots from the last one down, eight bytes at a time (the final
+// partial word at the top of the next slot), and the four lanes of a shard,
+// which advance nearly in step, fill each slot's 32 bytes together.
+constexpr int kLaneSlots = kLaneBytes / 8;  // 192
 constexpr int kScratchBytes = kScratchLanes + 4 * kLaneBytes;  // 9216
 
 // Order-0 rank bytes, by group of kRansShards consecutive shards.  rANS lane

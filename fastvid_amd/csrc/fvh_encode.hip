@@ -1135,7 +1135,6 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   int nbytes = 0;
   uint32_t x = kRansL;
   const int nsym = act && count > r ? (count - r + 3) >> 2 : 0;  // symbols of this lane
-  uint8_t* lane_end = sbase + kScratchLanes + (r + 1) * kLaneBytes;
   // ranks: the lane's bytes start at byte 1024 - nsym of its 1024; aligned
   // chunks ci, ci+1 in a, b (chunk i at cp[64 i]: the group's lanes'
   // chunks interleave); a block of 16 symbols takes the 16 bytes at offset
@@ -1157,13 +1156,15 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   uint32_t pbits = 0, stored = 0;
   // byte blocks go through a buffer resource over the group's 16 scratch
   // areas; rounds with nothing to store use an offset past it, which the
-  // hardware drops (a shared sink line would take every wave's stores)
+  // hardware drops (a shared sink line would take every wave's stores).
+  // Word w of the lane goes to slot kLaneSlots - 1 - w of the shard's
+  // interleaved lane area (lane r at byte 8 r of each 32-byte slot).
   const uint32_t grp_bytes =
       (uint32_t)(min(shard_total - grp * kRansShards, kRansShards) * kScratchBytes);
   const __amdgpu_buffer_rsrc_t lanes_r = __builtin_amdgcn_make_buffer_rsrc(
       scratch + (int64_t)grp * kRansShards * kScratchBytes, 0, (int)grp_bytes, 0x00020000);
-  const uint32_t lane_end_off =
-      (uint32_t)((lane >> 2) * kScratchBytes + kScratchLanes + (r + 1) * kLaneBytes);
+  const uint32_t lane_top_off =
+      (uint32_t)((lane >> 2) * kScratchBytes + kScratchLanes + kLaneSlots * 32 + 8 * r);
   // rounds run unpredicated while every active lane of the wave has 16 more
   // symbols; inactive lanes compute nothing of use and store nothing
   // (a group without active lanes runs no rounds at all)
@@ -1221,7 +1222,7 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
       pbits = full ? d : tb;
       const uint32_t next = stored + 8u;
       const bool fits = next <= (uint32_t)kLaneBytes;
-      const uint32_t off = full && fits && act ? lane_end_off - next : 0x80000000u;
+      const uint32_t off = full && fits && act ? lane_top_off - 4u * next : 0x80000000u;
       using u32x2 = uint32_t __attribute__((ext_vector_type(2)));
       __builtin_amdgcn_raw_buffer_store_b64(u32x2{(uint32_t)word, (uint32_t)(word >> 32)}, lanes_r,
                                             off, 0, 0);
@@ -1237,13 +1238,13 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   for (; j0 < full_j; j0 += 16) block(j0, false);
   for (; j0 < wave_j; j0 += 16) block(j0, true);
   if (act) {
-    // the last pend (< 8) bytes: byte i of the queue (i = 0 most recent) is
-    // emitted byte stored + pend - 1 - i, stored at lane_end - 1 - that
+    // the last pend (< 8) bytes, most recent lowest in the queue, go to the
+    // top of the next slot: they precede the stored words in the lane
     const int pend = (int)(pbits >> 3);
     nbytes = (int)stored + pend;
-    for (int i = 0; i < pend; ++i) {
-      const int e_i = (int)stored + pend - 1 - i;
-      if (e_i < kLaneBytes) lane_end[-1 - e_i] = (uint8_t)(qq >> (8 * i));
+    if (pend > 0 && (int)stored < kLaneBytes) {
+      *(uint64_t*)(sbase + kScratchLanes + 32 * (kLaneSlots - 1 - (int)(stored >> 3)) + 8 * r) =
+          qq << ((64u - pbits) & 63u);
     }
     if (nbytes > kLaneBytes) set_status(st, kStatusLaneOverflow);
     recs[s].states[r] = x;
@@ -1655,15 +1656,16 @@ __global__ __launch_bounds__(256) void k_emit(const FrameDesc* __restrict__ fram
 // with no LDS staging: the record is [mode][body u16][sparse table] at
 // kScratchHead, then the lane lengths 0..2 and final states at
 // kScratchLensStates (both written by k_rans), then the four lanes' bytes
-// at the ends of their areas.  kEmitO0Lanes (32) lanes per record, two
+// from the interleaved lane area.  kEmitO0Lanes (32) lanes per record, two
 // records per wave (16 lanes took 3 % longer on 4K RGB records of about a
 // kilobyte, 64 lanes 13 %); lane j of a record writes destination 16-byte
-// chunks j, j + kEmitO0Lanes, ....  A chunk takes, from
-// each piece it overlaps (usually one), sixteen bytes loaded as five dwords
-// and byte-aligned, merged under a byte mask; only the chunks at the
-// record's two ends, shared with the neighbouring records, store bytes.
-// The loads reach at most 15 bytes (plus a dword's remainder) outside a
-// piece, which the scratch layout keeps inside the allocation.
+// chunks j, j + kEmitO0Lanes, ....  A chunk takes, from each piece it
+// overlaps (usually one), the sixteen bytes at its position: three 8-byte
+// words of the piece (consecutive for the head and lengths, 32 bytes apart
+// for a lane), byte-aligned and merged under a byte mask; only the chunks at
+// the record's two ends, shared with the neighbouring records, store bytes.
+// The head and lengths reach at most 15 bytes before and 23 after a piece,
+// inside the shard's scratch; a lane's slots are clamped to its area.
 __device__ __forceinline__ uint32_t byte_span_mask(int lo, int hi, int d) {
   // bytes [lo, hi) of a 16-byte chunk, restricted to its dword d
   const int l = clampi(lo - 4 * d, 0, 4);
@@ -1682,7 +1684,6 @@ __global__ __launch_bounds__(256) void k_emit_o0(const FrameDesc* __restrict__ f
                                                  int shard_total, const ShardRec* __restrict__ recs,
                                                  const uint8_t* __restrict__ scratch,
                                                  const int64_t* __restrict__ tile_off, char* gbase) {
-  using u4a = uint4 __attribute__((aligned(4)));
   const int j = (int)threadIdx.x % kEmitO0Lanes;
   const int s = (int)(blockIdx.x * (256 / kEmitO0Lanes) + threadIdx.x / kEmitO0Lanes);
   if (s >= shard_total) return;
@@ -1719,18 +1720,28 @@ __global__ __launch_bounds__(256) void k_emit_o0(const FrameDesc* __restrict__ f
       const int lo = max(tlo, sel(p, 0, e0, e1, e2, e3, e4) - o0);
       const int hi = min(thi, sel(p, e0, e1, e2, e3, e4, total) - o0);
       if (lo >= hi) continue;  // an empty piece (a lane without bytes)
-      const int off = sel(p, kScratchHead, kScratchLensStates - e0,
-                          kScratchLanes + 1 * kLaneBytes - e2, kScratchLanes + 2 * kLaneBytes - e3,
-                          kScratchLanes + 3 * kLaneBytes - e4, kScratchLanes + 4 * kLaneBytes - total);
-      const uint8_t* src = sb + off + o0;
-      const uint32_t k = (uint32_t)((uintptr_t)src & 3);
-      const uint32_t* w = (const uint32_t*)(src - k);
-      const uint4 q = *(const u4a*)w;
-      const uint32_t q4 = w[4];
-      const uint32_t x0 = __builtin_amdgcn_alignbyte(q.y, q.x, k);
-      const uint32_t x1 = __builtin_amdgcn_alignbyte(q.z, q.y, k);
-      const uint32_t x2 = __builtin_amdgcn_alignbyte(q.w, q.z, k);
-      const uint32_t x3 = __builtin_amdgcn_alignbyte(q4, q.w, k);
+      // piece byte o0 + t is byte vr + t of the piece's run: scratch byte
+      // vr + t of the head and lengths (slots of 8), virtual lane byte of a
+      // lane (slots of 32)
+      const bool lane_piece = p >= 2;
+      const int vr = o0 + sel(p, kScratchHead, kScratchLensStates - e0, kLaneBytes - e2,
+                             kLaneBytes - e3, kLaneBytes - e4, kLaneBytes - total);
+      const uint8_t* const run = lane_piece ? sb + kScratchLanes + 8 * (p - 2) : sb;
+      const int stride = lane_piece ? 32 : 8;
+      const int slot = vr >> 3;
+      auto word = [&](int t) __attribute__((always_inline)) {
+        const int sl = lane_piece ? clampi(slot + t, 0, kLaneSlots - 1) : slot + t;
+        return *(const uint2*)(run + stride * sl);
+      };
+      const uint2 w0 = word(0), w1 = word(1), w2 = word(2);
+      const uint32_t k = (uint32_t)(vr & 3);
+      const bool up = (vr & 4) != 0;
+      const uint32_t d0 = up ? w0.y : w0.x, d1 = up ? w1.x : w0.y, d2 = up ? w1.y : w1.x;
+      const uint32_t d3 = up ? w2.x : w1.y, d4 = up ? w2.y : w2.x;
+      const uint32_t x0 = __builtin_amdgcn_alignbyte(d1, d0, k);
+      const uint32_t x1 = __builtin_amdgcn_alignbyte(d2, d1, k);
+      const uint32_t x2 = __builtin_amdgcn_alignbyte(d3, d2, k);
+      const uint32_t x3 = __builtin_amdgcn_alignbyte(d4, d3, k);
       const uint32_t m0 = byte_span_mask(lo, hi, 0), m1 = byte_span_mask(lo, hi, 1);
       const uint32_t m2 = byte_span_mask(lo, hi, 2), m3 = byte_span_mask(lo, hi, 3);
       v.x = (x0 & m0) | (v.x & ~m0);
