@@ -1046,9 +1046,9 @@ __global__ __launch_bounds__(kFileThreads) void k_o0_file(int32_t* __restrict__ 
 // (decode.cu:519-527).
 //
 // The symbol loop runs in rounds of four steps with uniform control flow: a
-// step renormalises with at most two bytes (a decoded state is >= 2^11), so a
-// round needs at most eight, taken big-endian from a window of three chunks
-// of the lane's bytes (see `iteration`).  While every lane of the wave has
+// step renormalises with at most two bytes (a decoded state is >= 2^11), so
+// an iteration of two rounds needs at most sixteen, taken big-endian from
+// the lane's two current chunks (see `iteration`).  While every lane of the wave has
 // four symbols left the rounds run unpredicated; a tail loop finishes lanes of
 // uneven length.  Decoded values go to a stage per shard in LDS (O0Block)
 // and leave every two rounds as one 16-byte store per lane (16-bit output;
@@ -1088,15 +1088,21 @@ __device__ __forceinline__ Chunk load_chunk(const uint4* p) {
   return Chunk{((uint64_t)v.y << 32) | v.x, ((uint64_t)v.w << 32) | v.z};
 }
 
-// Eight bytes starting at byte `pos` (0..15) of the 32-byte window A:B, the
-// first byte most significant, as a funnel shift of two quadwords.
-__device__ __forceinline__ uint64_t window8_be(uint64_t a0, uint64_t a1, uint64_t b0, int pos) {
+// Sixteen bytes starting at byte `pos` (0..15) of the 32-byte window A:B,
+// the first byte most significant: bytes pos..pos+7 returned, bytes
+// pos+8..pos+15 in `next`, each a funnel shift of two quadwords.
+__device__ __forceinline__ uint64_t window16_be(uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1,
+                                                int pos, uint64_t& next) {
   const bool low = pos < 8;
-  const uint64_t lo = low ? a0 : a1;
-  const uint64_t hi = low ? a1 : b0;
+  const uint64_t q0 = low ? a0 : a1;
+  const uint64_t q1 = low ? a1 : b0;
+  const uint64_t q2 = low ? b0 : b1;
   const uint32_t sh = 8u * (uint32_t)(pos & 7);
-  const uint64_t le = sh ? (lo >> sh) | (hi << (64u - sh)) : lo;
-  return __builtin_bswap64(le);
+  // (q << 1) << (63 - sh): q << (64 - sh), and nothing for sh = 0
+  const uint64_t le0 = (q0 >> sh) | ((q1 << 1) << (63u - sh));
+  const uint64_t le1 = (q1 >> sh) | ((q2 << 1) << (63u - sh));
+  next = __builtin_bswap64(le1);
+  return __builtin_bswap64(le0);
 }
 
 // popcount(a) + b in one instruction (the compiler otherwise distributes the
@@ -1189,7 +1195,7 @@ __device__ __forceinline__ void o0_lanes(const Lookup& look, int item, int nsym,
                                          const int32_t* __restrict__ lists,
                                          FrameState* __restrict__ states,
                                          FT* __restrict__ folded, FT* const stage, char* gbase,
-                                         uint8_t* __restrict__ sink, const void* shared_line,
+                                         uint8_t* __restrict__ sink,
                                          bool fair = false) {
   DShard d = {};
   if (item >= 0) d = dshards[lists[item]];
@@ -1206,11 +1212,9 @@ __device__ __forceinline__ void o0_lanes(const Lookup& look, int item, int nsym,
   const Chunk A0 = load_chunk(cp);
   const Chunk B0 = load_chunk(cp + min(1, last));
   const Chunk C0 = load_chunk(cp + min(2, last));
-  const Chunk D0 = load_chunk(cp + min(3, last));
-  uint64_t a0 = A0.lo, a1 = A0.hi, b0 = B0.lo, b1 = B0.hi, c0 = C0.lo, c1 = C0.hi;
-  uint64_t d0 = D0.lo, d1 = D0.hi;  // chunk ci + 3, or in flight into n0:n1
-  uint64_t n0 = 0, n1 = 0;
-  bool pend = false;   // n0:n1 holds chunk ci + 3 once its load lands
+  uint64_t a0 = A0.lo, a1 = A0.hi, b0 = B0.lo, b1 = B0.hi;
+  uint64_t c0 = C0.lo, c1 = C0.hi;  // chunk ci + 2 (its load may be in flight)
+  bool pend = false;   // the window moved: chunk ci + 2 is to be loaded into C
   int ci = 0;          // chunk index of A
   int pos = head;      // next byte within A:B, < 16 when an iteration starts
   int consumed = 0;    // lane bits consumed
@@ -1270,56 +1274,53 @@ __device__ __forceinline__ void o0_lanes(const Lookup& look, int item, int nsym,
       for (int j = 0; p0 + j < n; ++j) out[p0 + j] = stage[kPer * r + j];
     }
   };
-  // Two rounds: the first reads A:B (pos < 16), the second A:B:C (pos < 24).
-  // After them the window moves by one chunk at most (C <- D); a lane that
-  // moved loads the chunk after D into n0:n1 as the iteration's last memory
-  // operation (`refill`), and the next iteration takes it into D at its
-  // end, so each load has a whole iteration to land.
+  // Eight steps take at most 16 bytes: an iteration reads the 16 bytes at
+  // pos of A:B, in two rounds of four steps (the bit window of a round is
+  // 64 bits; the second round's continues with the bytes the first did not
+  // reach).  After them the window moves by one chunk at most (B <- C); a
+  // lane that moved loads the chunk after B into C as the iteration's last
+  // memory operation (`refill`), which the next move takes, so each load
+  // has a whole iteration to land.
   // (h8: the iteration's first step within its block)
   auto iteration = [&](int s0, int h8, bool predicated) __attribute__((always_inline)) {
-    int used_total = 0;
-    {
-      uint64_t bb = window8_be(a0, a1, b0, pos);
-      int used = 0;
+    uint64_t rest;
+    uint64_t bb = window16_be(a0, a1, b0, b1, pos, rest);
+    int used = 0;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) step(bb, used, h8 + t, !predicated || s0 + t < nsteps);
-      used_total = used;
-    }
+    for (int t = 0; t < 4; ++t) step(bb, used, h8 + t, !predicated || s0 + t < nsteps);
     {
-      const int p1 = pos + (used_total >> 3);
-      const bool hi_half = p1 >= 16;
-      uint64_t bb = window8_be(hi_half ? b0 : a0, hi_half ? b1 : a1, hi_half ? c0 : b0, p1 & 15);
-      int used = 0;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        step(bb, used, h8 + 4 + t, !predicated || s0 + 4 + t < nsteps);
-      }
-      used_total += used;
+      // bb is the first 8 bytes << used (0..64 bits): append the next
+      // `used` bits, rest >> (64 - used), as two shifts of 32 - used / 2
+      const uint32_t half = 32u - ((uint32_t)used >> 1);
+      bb |= (rest >> half) >> half;
     }
-    consumed += used_total;
-    pos += used_total >> 3;
-    const bool adv = pos >= 16;
-    // (kept after the rounds: the wait for last iteration's load goes here)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) step(bb, used, h8 + 4 + t, !predicated || s0 + 4 + t < nsteps);
+    consumed += used;
+    const int p = pos + (used >> 3);  // < 32
+    const bool adv = p >= 16;
+    // (kept after the rounds: the wait for the last refill goes here)
     __builtin_amdgcn_sched_barrier(0);
-    d0 = pend ? n0 : d0;
-    d1 = pend ? n1 : d1;
     a0 = adv ? b0 : a0;
     a1 = adv ? b1 : a1;
     b0 = adv ? c0 : b0;
     b1 = adv ? c1 : b1;
-    c0 = adv ? d0 : c0;
-    c1 = adv ? d1 : c1;
-    pos -= adv ? 16 : 0;
-    ci += adv ? 1 : 0;
+    // (the moves happen here, before the refill's branch: moved past it,
+    // they would keep C's old value alive under the load into C)
+    asm volatile("" : "+v"(a0), "+v"(a1), "+v"(b0), "+v"(b1));
+    pos = p & 15;
+    ci += p >> 4;
     pend = adv;
   };
-  // (lanes that did not move read one line every such lane of the wave
-  // shares, so the load stays unconditional and n0:n1 needs no merge)
+  // (only lanes that moved load, straight into C: the others keep C, which
+  // has landed, and lanes that do not load touch no memory)
   auto refill = [&]() __attribute__((always_inline)) {
     __builtin_amdgcn_sched_barrier(0);
-    const Chunk Nn = load_chunk(pend ? cp + min(ci + 3, last) : (const uint4*)shared_line);
-    n0 = Nn.lo;
-    n1 = Nn.hi;
+    if (pend) {
+      const Chunk Nn = load_chunk(cp + min(ci + 2, last));
+      c0 = Nn.lo;
+      c1 = Nn.hi;
+    }
     __builtin_amdgcn_sched_barrier(0);
   };
 
@@ -1429,7 +1430,7 @@ __global__ __launch_bounds__(64) void k_dec_o0(const FrameDesc* __restrict__ fra
     // ---- the shard's rANS lanes
     BucketLookup<FT, kMax> look{bucket, ent};
     o0_lanes<FT, false>(look, item, nsym, tab, r, lane, frames, dshards, lists, states, folded,
-                        stage, gbase, sink, tabs, fair != 0);
+                        stage, gbase, sink, fair != 0);
     int next = 0;
     if (lane == 0) next = atomicAdd(queue, 1);
     grp = (int)gridDim.x + __builtin_amdgcn_readfirstlane(next);
@@ -1515,7 +1516,7 @@ __global__ __launch_bounds__(64) void k_dec_o0_direct(const FrameDesc* __restric
     __builtin_amdgcn_wave_barrier();
     DirectLookup look{slot, val};
     o0_lanes<FT, true>(look, item, nsym, tab, r, lane, frames, dshards, lists, states, folded,
-                       (FT*)lds.stage[g], gbase, sink, tabs);
+                       (FT*)lds.stage[g], gbase, sink);
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -1528,9 +1529,8 @@ __global__ __launch_bounds__(64) void k_dec_o0_direct(const FrameDesc* __restric
 // the one, then k remainder bits) decodes in one step.  Refills take the
 // bytes at `pos` of the lane's two-chunk window A:B: a round of four steps
 // reads at most 16 bytes from pos < 16, so the window moves by one chunk at
-// most per round, through the same chunk pipeline as k_dec_o0 (C, D behind
-// it, the next chunk loaded as the round's last memory operation by lanes
-// that moved).  Decoded values are staged per shard in LDS and leave as one
+// most per round, through a chunk pipeline (C, D behind it, the next chunk
+// loaded as the round's last memory operation by lanes that moved).  Decoded values are staged per shard in LDS and leave as one
 // 16-byte store per lane every two rounds.  A longer code (a zero run past
 // the buffer) is read from the lane's chunks in memory and the window is
 // reloaded where it ends; after that the round's remaining refills still
