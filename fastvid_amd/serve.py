@@ -161,13 +161,16 @@ class Server:
                      output=self.pool.view(off + o_out, max(job.output_bytes, 1)),
                      results=self.pool.view(off + o_res, len(batch) * _native.RESULT_BYTES))
         stream = self.streams[slot]
+        # the region may have held a retired batch whose kept outputs are
+        # still being copied out on the current stream (_retire enqueues the
+        # copies, then releases the region): nothing is written to it before
+        # those copies ran
+        stream.wait_stream(torch.cuda.current_stream(self.device))
         ready = [r.ready for r in batch if r.ready is not None]
         if ready:
             # frames come from one in-order stream and a batch holds requests
             # in arrival order: the last one's event covers every frame
             stream.wait_event(ready[-1])
-        else:
-            stream.wait_stream(torch.cuda.current_stream(self.device))
         job.launch(stream=stream)
         ev = torch.cuda.Event()
         ev.record(stream)
@@ -192,6 +195,10 @@ class Server:
         sizes = self.pool.view(off + o_sz, 8 * len(batch)).view(torch.int64)
         res = self.pool.view(off + o_res, len(batch) * _native.RESULT_BYTES)
         stream = self.streams[slot]
+        # before anything lands in the region: the copies of a retired
+        # batch's kept outputs out of it, and the copies that made the input
+        # streams, were enqueued on the current stream
+        stream.wait_stream(torch.cuda.current_stream(self.device))
         # the sizes go up asynchronously from pinned memory on the slot's
         # stream (a synchronous copy would stall the request loop); the host
         # buffer stays referenced by the batch until it retires
@@ -202,7 +209,6 @@ class Server:
         ins = (ctypes.c_void_p * len(batch))(*[r.stream.data.data_ptr() for r in batch])
         outs = (ctypes.c_void_p * len(batch))(*[o.data_ptr() for o in outputs])
         msg = _native.message_buffer()
-        stream.wait_stream(torch.cuda.current_stream(self.device))
         _native.check(_native.lib().fvh_decode_launch_sized(
             ctypes.addressof(infos), ins, ctypes.c_void_p(sizes.data_ptr()), outs, len(batch),
             ws.data_ptr(), ws.numel(), res.data_ptr(), 1, ctypes.c_void_p(stream.cuda_stream),
@@ -236,6 +242,8 @@ class Server:
                 if res.status == 0 and (self.keep_outputs or r.rid % self.keep_every == 0):
                     r.output = outputs[i].clone()
                 r.done = now
+        # (the copies above are enqueued on the current stream; a batch given
+        # this region later waits for that stream before it writes)
         self.pool.release(*region)
         self.completed.extend(batch)
         self.busy[slot] = None

@@ -44,6 +44,21 @@ def test_served_workload_at_2000_per_second_equals_plain_calls():
     assert server.pool.used == 0
 
 
+def test_kept_outputs_survive_region_reuse_at_8000_per_second():
+    """At 8000 requests/s a retired batch's pool region is handed to the
+    next batch at once, while the copies of its kept outputs (enqueued on
+    the current stream when it retires) may not have run yet: the next
+    batch's stream must wait for them before it writes the region.  Every
+    output is kept and checked against the plain call."""
+    from fastvid_amd.serve import run, summarize
+    requests, elapsed, server = run(seconds=0.5, rate=8000.0, pool_bytes=64 << 30, slots=2,
+                                    max_batch=32, keep_outputs=True, seed=23)
+    assert len(requests) > 3000
+    _check(requests)
+    assert summarize(requests, elapsed)["failed"] == 0
+    assert server.pool.used == 0
+
+
 def test_outputs_alone_equal_outputs_in_the_mix():
     from fastvid_amd.serve import run, summarize
     requests, elapsed, server = run(seconds=0.5, rate=300.0, pool_bytes=4 << 30, slots=2,
