@@ -961,8 +961,11 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
     int best_k = 0;
     int64_t best_bytes = INT64_MAX;
     uint32_t best_lane[4] = {0, 0, 0, 0};
+    // from k = bl on every x >> k is 0, so a lane's cost n (1 + k) bits only
+    // grows with k, and the strict comparison keeps the first of equal
+    // costs: no k past bl can be chosen
 #pragma unroll 1
-    for (int k = 0; k <= kMaxRice; ++k) {
+    for (int k = 0; k <= min(bl, kMaxRice); ++k) {
       uint32_t cs[4] = {0, 0, 0, 0};
       if (k < bl) {
         if (exact) {
@@ -1058,8 +1061,9 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
 // rank layout makes each such load one contiguous kilobyte for the wave.  A step's
 // renormalisation bytes go into the round's 64-bit word, a round (four
 // steps) merges them into a queue of fewer than eight bytes, and when eight
-// are queued it stores the oldest eight to the end of the lane's scratch
-// area, backwards, which thus ends with the lane's bytes in decode order;
+// are queued it stores the oldest eight to the next slot of the lane's
+// column in the shard's interleaved lane area, from the top down, which thus
+// ends with the lane's bytes in decode order;
 // rounds with nothing to store issue a store the hardware drops, so control
 // flow and memory waits are the same every round.  Rounds run unpredicated
 // while every active lane has 16 symbols left.  The last step decides the
