@@ -111,6 +111,8 @@ def lib():
     handle.fvh_results_copy.argtypes = [p, p, i32, p, cs, sz]
     handle.fvh_set_band_waves.restype = ctypes.c_int
     handle.fvh_set_band_waves.argtypes = [i32]
+    handle.fvh_set_band_waves_decode.restype = ctypes.c_int
+    handle.fvh_set_band_waves_decode.argtypes = [i32]
     handle.fvh_synth_plane.restype = ctypes.c_int
     handle.fvh_synth_plane.argtypes = [p, p, p]
     handle.fvh_build_info.restype = ctypes.c_char_p
@@ -142,9 +144,18 @@ def timing_read(reset: bool = True) -> dict:
 
 
 def band_waves(set_to: int = 0) -> int:
-    """Persistent waves per compute unit of the band wavefront; ``set_to``
-    > 0 changes it for launches made afterwards.  Returns the value in force."""
+    """Persistent waves per compute unit of the predictor's band wavefront;
+    ``set_to`` > 0 changes it for launches made afterwards.  Returns the
+    value in force."""
     value = lib().fvh_set_band_waves(int(set_to))
+    if value < 0:
+        raise ValueError("band waves per compute unit must be in 1..64")
+    return value
+
+
+def band_waves_decode(set_to: int = 0) -> int:
+    """The same for the reconstruction's band wavefront (default 4)."""
+    value = lib().fvh_set_band_waves_decode(int(set_to))
     if value < 0:
         raise ValueError("band waves per compute unit must be in 1..64")
     return value

@@ -803,6 +803,11 @@ int fvh_set_band_waves(int32_t waves_per_cu) {
   return fvh::band_waves_per_cu(waves_per_cu);
 }
 
+int fvh_set_band_waves_decode(int32_t waves_per_cu) {
+  if (waves_per_cu < 0 || waves_per_cu > 64) return -1;
+  return fvh::band_decode_waves_per_cu(waves_per_cu);
+}
+
 const char* fvh_build_info(void) { return "fastvid_hip 0.1 gfx950"; }
 
 }  // extern "C"

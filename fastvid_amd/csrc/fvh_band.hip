@@ -665,10 +665,16 @@ __global__ __launch_bounds__(64) void k_band(const FrameDesc* __restrict__ frame
 }
 
 static std::atomic<int> g_band_waves_per_cu{kBandWavesPerCuDefault};
+static std::atomic<int> g_band_decode_waves_per_cu{kBandDecodeWavesPerCuDefault};
 
 int band_waves_per_cu(int set) {
   if (set > 0) g_band_waves_per_cu.store(set);
   return g_band_waves_per_cu.load();
+}
+
+int band_decode_waves_per_cu(int set) {
+  if (set > 0) g_band_decode_waves_per_cu.store(set);
+  return g_band_decode_waves_per_cu.load();
 }
 
 template <bool kEncode, typename FT>
@@ -679,7 +685,8 @@ void launch_band(const FrameDesc* frames, FrameState* states, const TileDesc* ti
   const int lds = band_lds_bytes<kEncode, FT>(ring_groups);
   // persistent waves (a tunable count per compute unit): each takes tile
   // groups of four tiles from the queue until none are left
-  const dim3 grid(min(group_total, band_waves_per_cu(0) * device_cus()));
+  const int waves = kEncode ? band_waves_per_cu(0) : band_decode_waves_per_cu(0);
+  const dim3 grid(min(group_total, waves * device_cus()));
   if (med) {
     FVH_TIMED(kEncode ? "k_band_enc" : "k_band_dec", stream);
     hipLaunchKernelGGL((k_band<kEncode, true, FT>), grid, dim3(64), lds, stream, frames, states,

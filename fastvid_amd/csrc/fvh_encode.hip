@@ -585,7 +585,8 @@ __device__ __forceinline__ ClampFn clamp_scan(ClampFn g) {
 // of each wave's phases, so the copies trade atomic collisions against waves
 // per compute unit: on 16 4K RGB frames (most shards on the private
 // histogram below) 4 copies and 9 waves took 319 us, 3 and 11 waves 278 us,
-// 2 and 17 waves 250 us.
+// 2 and 16 waves 250 us; sharing the rank map's bytes with the zero words
+// (18 waves) 233 us.
 constexpr int kHistCopies = 2;
 constexpr int kHistRow = 512 + 8;
 // Shards whose values are all below kSmallBins (most of them: the residuals

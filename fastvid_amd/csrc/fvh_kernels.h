@@ -98,11 +98,16 @@ constexpr uint64_t kStaticWorkspaceBytes = 256ull << 20;
 // device with hipGetSymbolAddress, then cached).
 hipError_t static_workspace(hipStream_t stream, void** out);
 
-// Persistent waves per compute unit of the band wavefront (k_band): the
-// launch-geometry knob the autotuner sets beside the tile size.  set > 0
-// changes it for later launches; returns the current value.
+// Persistent waves per compute unit of the band wavefront (k_band), the
+// predictor's (the launch-geometry knob the autotuner sets beside the tile
+// size) and the reconstruction's.  set > 0 changes it for later launches;
+// returns the current value.  The reconstruction is faster with fewer
+// streams of tile rows in flight (16 4K RGB frames: 325 us at 4 waves, 339
+// at 8, 341 at 5, 349 at 6), the predictor with more (268 us at 8).
 constexpr int kBandWavesPerCuDefault = 8;
+constexpr int kBandDecodeWavesPerCuDefault = 4;
 int band_waves_per_cu(int set);
+int band_decode_waves_per_cu(int set);
 
 void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,
                  int32_t* group_first, int tile_total, hipStream_t stream);

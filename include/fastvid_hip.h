@@ -173,14 +173,19 @@ typedef struct fvh_kernel_time {
 int fvh_timing_enable(int32_t enable);
 int fvh_timing_read(fvh_kernel_time* out, int32_t max, int32_t reset);
 
-/* Launch geometry of the predictor / reconstruction wavefront (no
- * counterpart in the reference, whose predict_tiles_kernel runs one block
- * per tile, cuda/csrc/encode.cu:780-783): persistent waves per compute unit,
- * each taking groups of four tiles (16 lanes per tile) from a device queue.
+/* Launch geometry of the predictor wavefront (no counterpart in the
+ * reference, whose predict_tiles_kernel runs one block per tile,
+ * cuda/csrc/encode.cu:780-783): persistent waves per compute unit, each
+ * taking groups of four tiles (16 lanes per tile) from a device queue.
  * 1..64 sets it for launches enqueued afterwards (graphs keep the value they
  * were captured with); 0 only reads it.  Returns the value in force
  * (default 8), or -1 for an argument outside 0..64. */
 int fvh_set_band_waves(int32_t waves_per_cu);
+
+/* The same for the decoder's reconstruction wavefront (no counterpart in
+ * the reference, whose reconstruct_tiles_kernel runs one block per tile,
+ * cuda/csrc/decode.cu:696-697); default 4. */
+int fvh_set_band_waves_decode(int32_t waves_per_cu);
 
 /* Seeded synthetic plane on the device (no counterpart in the reference;
  * the serving workload generates each request's frame with it).  Writes

@@ -124,3 +124,25 @@ def test_band_wave_counts_give_the_same_streams(batch, waves):
     finally:
         _native.band_waves(waves0)
     assert _native.band_waves() == waves0
+
+
+@pytest.mark.parametrize("waves", (1, 8, 24))
+def test_reconstruction_wave_counts_give_the_same_planes(batch, waves):
+    """The reconstruction's own wave count (default 4) leaves every decoded
+    plane as it is at the default."""
+    from fastvid_amd import _native, codec
+    specs, host, dev = batch
+    kw = [dict(layout=s["layout"], bit_depth=s["bit_depth"], quality=s["quality"])
+          for s in specs[:12]]
+    args = [codec.frame_args(f, **k) for f, k in zip(dev[:12], kw)]
+    streams = [t.clone() for t in codec.encode_batch(args)]
+    refd = [tuple(p.clone() for p in f) for f in codec.decode_batch(streams)]
+    waves0 = _native.band_waves_decode()
+    assert waves0 == 4
+    try:
+        assert _native.band_waves_decode(waves) == waves
+        for fa, fb in zip(codec.decode_batch(streams), refd):
+            assert all(torch.equal(a, b) for a, b in zip(fa, fb))
+    finally:
+        _native.band_waves_decode(waves0)
+    assert _native.band_waves_decode() == waves0
