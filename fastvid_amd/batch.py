@@ -300,8 +300,9 @@ def autotune(specs: Sequence[FrameSpec], frames: Sequence, tile_candidates=TILE_
              wave_candidates=WAVE_CANDIDATES, reps: int = 5, objective: str = "encode+decode",
              per_format: bool = True, margin: float = 0.01, device=None,
              progress=None) -> TuneResult:
-    """Tile geometry per frame format and band waves per compute unit that
-    minimise the graph replay time of the whole batch.
+    """Tile geometry per frame format and the predictor's band waves per
+    compute unit (the reconstruction keeps its own count) that minimise the
+    graph replay time of the whole batch.
 
     The objective is one replay of a ``GraphEncoder`` over every frame of the
     batch (and, for "encode+decode", a ``GraphDecoder`` of its streams).  The
