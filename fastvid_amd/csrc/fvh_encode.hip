@@ -1100,7 +1100,8 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
                                            ShardRec* __restrict__ recs,
                                            uint8_t* __restrict__ scratch,
                                            const uint8_t* __restrict__ rank_area, uint4* ent,
-                                           int grp, int cap, int cap_below) {
+                                           int grp, int cap, int cap_below,
+                                           uint8_t* __restrict__ tier_of) {
   using q2 = uint64_t __attribute__((ext_vector_type(2)));
   const int lane = lane_id();
   const int r = lane & 3;
@@ -1121,6 +1122,9 @@ __device__ __forceinline__ void rans_group(const FrameDesc* __restrict__ frames,
   // the group belongs to the instance whose capacity range holds its largest
   // alphabet
   const int largest = __builtin_amdgcn_readfirstlane((int)wave_max((uint32_t)(act ? distinct : 0)));
+  // the first instance (every group) notes each group's capacity tier for
+  // the others, which then visit their own groups only
+  if (cap_below < 0 && lane == 0) tier_of[grp] = (uint8_t)(largest > 64 ? 2 : largest > 16 ? 1 : 0);
   if (largest > cap || largest <= cap_below) return;
   uint8_t* sbase = scratch + (int64_t)(act ? s : 0) * kScratchBytes;
   if (act) {
@@ -1298,16 +1302,33 @@ __global__ __launch_bounds__(64) void k_rans(const FrameDesc* __restrict__ frame
                                              ShardRec* __restrict__ recs,
                                              uint8_t* __restrict__ scratch,
                                              const uint8_t* __restrict__ rank_area,
-                                             int cap_below) {
+                                             int cap_below, uint8_t* __restrict__ tier_of) {
   (void)folded;
   // two entries of padding per row: the shards' rows start eight banks
   // apart, so the four shards a 16-lane pass of a 16-byte LDS read covers
   // take their two commonest ranks from distinct banks
   __shared__ uint4 ent_all[kRansShards][kCap + 2];
   const int groups = (shard_total + kRansShards - 1) / kRansShards;
-  for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
-    rans_group<FT>(frames, states, tiles, shards, shard_total, recs, scratch, rank_area,
-                   ent_all[lane_id() >> 2], grp, kCap, cap_below);
+  if (cap_below < 0) {
+    for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+      rans_group<FT>(frames, states, tiles, shards, shard_total, recs, scratch, rank_area,
+                     ent_all[lane_id() >> 2], grp, kCap, cap_below, tier_of);
+    }
+    return;
+  }
+  // the wider instances: a wave's next 64 groups' tiers in one load, then
+  // only the groups of its own tier (none, for small alphabets)
+  const uint8_t mine = kCap > 64 ? 2 : 1;
+  const int stride = (int)gridDim.x;
+  for (int base = blockIdx.x; base < groups; base += 64 * stride) {
+    const int g = base + lane_id() * stride;
+    uint64_t m = ballot(g < groups && tier_of[g] == mine);
+    while (m != 0) {
+      const int l = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      rans_group<FT>(frames, states, tiles, shards, shard_total, recs, scratch, rank_area,
+                     ent_all[lane_id() >> 2], base + l * stride, kCap, cap_below, tier_of);
+    }
   }
 }
 
@@ -1811,19 +1832,23 @@ static void encode_pass(const EncodeBuffers& b, int pass, hipStream_t stream) {
     FVH_TIMED("k_rans", stream);
     // one persistent instance per table capacity; each takes its groups
     const int groups = (b.shard_total + kRansShards - 1) / kRansShards;
+    // each group's capacity tier, one byte a group, in the emit list's
+    // bytes (the list is written only by the final tile scan, after every
+    // rANS pass)
+    uint8_t* const tier_of = (uint8_t*)b.emit_list;
     const dim3 grid(min(groups, 32 * device_cus()));
     hipLaunchKernelGGL((k_rans<FT, 16>), grid, dim3(64), 0, stream, b.frames, b.states, b.tiles,
-                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.ranks, -1);
+                       b.shards, b.shard_total, folded, b.recs, b.scratch, b.ranks, -1, tier_of);
     // the wider tables hold fewer waves per compute unit; their instances
     // stride over every group, so a grid of resident waves loses nothing
     static const int o64 = blocks_per_cu((const void*)k_rans<FT, 64>, 0);
     static const int o256 = blocks_per_cu((const void*)k_rans<FT, 256>, 0);
     hipLaunchKernelGGL((k_rans<FT, 64>), dim3(min(groups, o64 * device_cus())), dim3(64), 0,
                        stream, b.frames, b.states, b.tiles, b.shards, b.shard_total, folded,
-                       b.recs, b.scratch, b.ranks, 16);
+                       b.recs, b.scratch, b.ranks, 16, tier_of);
     hipLaunchKernelGGL((k_rans<FT, 256>), dim3(min(groups, o256 * device_cus())), dim3(64), 0,
                        stream, b.frames, b.states, b.tiles, b.shards, b.shard_total, folded,
-                       b.recs, b.scratch, b.ranks, 64);
+                       b.recs, b.scratch, b.ranks, 64, tier_of);
   }
 }
 
