@@ -1078,9 +1078,11 @@ __global__ __launch_bounds__(64) void k_analyze(const FrameDesc* __restrict__ fr
 //
 // Table capacity: a wave takes groups of 16 consecutive shards, and one
 // instance of the kernel per capacity (16, 64, 256 entries) takes the groups
-// whose largest alphabet falls in its range (the others skip them), so the
-// LDS held per wave follows the alphabets actually present: 4 KB for small
-// alphabets, which lets a compute unit hold every wave its registers allow.
+// whose largest alphabet falls in its range, so the LDS held per wave
+// follows the alphabets actually present: 4 KB for small alphabets, which
+// lets a compute unit hold every wave its registers allow.  The 16-entry
+// instance visits every group and notes each one's tier in a byte; the
+// wider instances read those bytes and visit their own groups only.
 
 // Eight bytes starting at byte `pos` (0..15) of the 32-byte window a0:a1:b0:b1,
 // the first byte least significant.
