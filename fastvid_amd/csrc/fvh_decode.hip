@@ -1,12 +1,20 @@
 // fvh_decode.hip -- MI355X decoder kernels.
 //
-//   k_dstate_init  per-frame status
-//   k_parse        directory + shard-header validation, one thread per tile
-//                  (checks of cuda/csrc/decode.cu:131-220)
-//   k_entropy      entropy decode of every mode, one wave64 per shard
-//                  (decode.cu:222-550)
-//   reconstruction: the shared wavefront kernel (fvh_encode.hip) or the
-//                  one-thread-per-tile diagnostic k_serial (decode.cu:608-656)
+//   (k_plan, fvh_plan.hip: tile geometry, frame states and counters)
+//   k_dheader      stream headers read on the device (device-fed sizes)
+//   k_parse        directory + shard-header validation, one wave per tile,
+//                  shards filed by entropy mode (decode.cu:131-220)
+//   k_o0_parse     order-0 tables, a thread per shard (k_o0_parse_wave +
+//                  k_o0_file: a wave per shard), filed by alphabet tier
+//   k_dec_o0       order-0 rANS, 16 shards per wave, one instance per tier
+//                  (k_dec_o0_direct: the latency form) (decode.cu:441-549)
+//   k_dec_rice     Rice shards, 16 per wave
+//   k_dec_other    zero-run and block-pack shards, and Rice shards the
+//                  reference rejects inside a long code (decode.cu:222-440)
+//   reconstruction: the band wavefront (fvh_band.hip), the shared wavefront
+//                  kernels (fvh_encode.hip) or the one-thread-per-tile
+//                  diagnostic k_serial (decode.cu:608-656)
+//   k_dresults     per-frame results
 #include "fvh_device.h"
 #include "fvh_kernels.h"
 #include "fvh_timing.h"
@@ -32,17 +40,6 @@ __host__ __device__ constexpr int o0_tier_cap(int t) { return t == 0 ? 16 : t ==
 constexpr int kRiceList = 2 + kO0Tiers;
 constexpr int kListCounts = kRiceList + 1;
 constexpr int kRiceQueue = 7;
-
-__global__ void k_dstate_init(FrameState* __restrict__ states, int frame_count,
-                              int32_t* __restrict__ list_count, int32_t* __restrict__ queues) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kListCounts) list_count[i] = 0;
-  if (i < kQueueCount) queues[i] = 0;
-  if (i >= frame_count) return;
-  FrameState s = {};
-  s.active = 1;
-  states[i] = s;
-}
 
 // Device-fed decode: the stream sizes are in device memory and each header is
 // read here, one thread per frame.  The header must describe the geometry the
@@ -1849,11 +1846,18 @@ template <typename FT>
 static void decode_all(const DecodeBuffers& b, hipStream_t stream) {
   const dim3 fgrid((b.frame_count + 63) / 64);
   {
-    FVH_TIMED("k_dstate_init", stream);
-    hipLaunchKernelGGL(k_dstate_init, dim3((max(b.frame_count, kQueueCount) + 63) / 64), dim3(64),
-                       0, stream, b.states, b.frame_count, b.list_count, b.queues);
+    // the tile geometry, and the frame states, list counts and work counters
+    // every kernel after it reads
+    PlanInit init = {};
+    init.kind = 2;
+    init.states = b.states;
+    init.queues = b.queues;
+    init.queue_count = kQueueCount;
+    init.list_count = b.list_count;
+    init.list_counts = kListCounts;
+    launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.group_first, b.tile_total, stream,
+                init);
   }
-  launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.group_first, b.tile_total, stream);
   if (b.fed_sizes != nullptr) {
     FVH_TIMED("k_dheader", stream);
     hipLaunchKernelGGL(k_dheader, fgrid, dim3(64), 0, stream, b.frames, b.states, b.frame_count,

@@ -2,7 +2,8 @@
 //
 // Pipeline (all device-side, one launch per stage for the whole batch, no host
 // synchronisation inside a call):
-//   k_state_init   per-frame pass state
+//   (k_plan, fvh_plan.hip: tile geometry, pass states and work counters)
+//   k_band         band wavefront predictor + quantiser (fvh_band.hip)
 //   k_wavefront    closed-loop predictor + quantiser, one wave64 per tile
 //   k_lossless     step-1 predictor (reconstruction == source: fully parallel)
 //   k_generic      wide-tile fallback (antidiagonal, recon in global memory)
@@ -46,18 +47,6 @@ void launch_upload_frames(const FrameDesc* host_frames, int frame_count, FrameDe
 }
 
 // ----------------------------------------------------------- pass control
-
-__global__ void k_state_init(const FrameDesc* __restrict__ frames, FrameState* __restrict__ states,
-                             int frame_count, int32_t* __restrict__ queues) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kQueueCount) queues[i] = 0;
-  if (i >= frame_count) return;
-  FrameState s = {};
-  s.step = frames[i].step0;
-  s.use_med = frames[i].layout != 0;  // first pass: MED for colour, gradient for gray (encode.cu:812)
-  s.active = 1;
-  states[i] = s;
-}
 
 // Between passes: the multi-pass heuristics of encode.cu:812-847.
 //   stage 1 (after the first pass): gray10 lossy re-runs with MED when the
@@ -1858,11 +1847,6 @@ template <typename FT>
 static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
   const dim3 fgrid((b.frame_count + 63) / 64);
   const dim3 tgrid((b.tile_total + 3) / 4);
-  {
-    FVH_TIMED("k_state_init", stream);
-    hipLaunchKernelGGL(k_state_init, dim3((max(b.frame_count, kQueueCount) + 63) / 64), dim3(64), 0,
-                       stream, b.frames, b.states, b.frame_count, b.queues);
-  }
   encode_pass<FT>(b, 0, stream);
   if (b.any_heuristic) {
     {
@@ -1931,7 +1915,15 @@ static void encode_all(const EncodeBuffers& b, hipStream_t stream) {
 }
 
 void launch_encode(const EncodeBuffers& b, hipStream_t stream) {
-  launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.group_first, b.tile_total, stream);
+  // the tile geometry, and the pass states and work counters every kernel
+  // after it reads
+  PlanInit init = {};
+  init.kind = 1;
+  init.states = b.states;
+  init.queues = b.queues;
+  init.queue_count = kQueueCount;
+  launch_plan(b.frames, b.frame_count, b.tiles, b.shards, b.group_first, b.tile_total, stream,
+              init);
   if (b.wide_folded) encode_all<uint32_t>(b, stream);
   else if (b.narrow_folded) encode_all<uint8_t>(b, stream);
   else encode_all<uint16_t>(b, stream);

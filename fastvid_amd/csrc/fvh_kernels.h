@@ -25,7 +25,7 @@ struct EncodeBuffers {
   uint8_t* ranks;        // kRankGroupBytes per group of kRansShards shards
   uint16_t* recon;       // tile-order reconstruction for tiles wider than kRingMax (or null)
   uint8_t* sink;         // kSinkBytes target of masked-off stores
-  int32_t* queues;       // kQueueCount work counters, zeroed by k_state_init
+  int32_t* queues;       // kQueueCount work counters, zeroed by k_plan
   int32_t* group_first;  // first tile of each band tile group (written by k_plan)
   int32_t* emit_list;    // shards whose records are not order-0 (final k_tile_scan),
                          // counted in queues[kEmitQueue]
@@ -60,7 +60,7 @@ struct DecodeBuffers {
   int32_t* list_count;   // [0] order-0, [1] zero-run / block-pack, [2 + t] tier t, [5] Rice
   O0Tab* o0tabs;         // parsed order-0 tables, by order-0 list position
   uint8_t* sink;         // kSinkBytes target of masked-off stores
-  int32_t* queues;       // kQueueCount work counters, zeroed by k_dstate_init
+  int32_t* queues;       // kQueueCount work counters, zeroed by k_plan
   int32_t* group_first;
   int group_total;
   int band_ring;
@@ -109,8 +109,21 @@ constexpr int kBandDecodeWavesPerCuDefault = 4;
 int band_waves_per_cu(int set);
 int band_decode_waves_per_cu(int set);
 
+// State a call starts from, set by the planning kernel's threads beside the
+// tile geometry (one launch fewer than a kernel of its own): the work
+// counters and list counts zeroed, each frame's state reset (the encoder's
+// with its first pass's step and predictor).
+struct PlanInit {
+  int kind;             // 0 nothing, 1 encoder states, 2 decoder states
+  FrameState* states;
+  int32_t* queues;      // queue_count work counters
+  int queue_count;
+  int32_t* list_count;  // list_counts counters (decoder)
+  int list_counts;
+};
 void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,
-                 int32_t* group_first, int tile_total, hipStream_t stream);
+                 int32_t* group_first, int tile_total, hipStream_t stream,
+                 const PlanInit& init = PlanInit{});
 void launch_upload_frames(const FrameDesc* host_frames, int frame_count, FrameDesc* device_frames,
                           hipStream_t stream);
 void launch_encode(const EncodeBuffers& b, hipStream_t stream);

@@ -18,8 +18,22 @@ namespace fvh {
 __global__ __launch_bounds__(256) void k_plan(const FrameDesc* __restrict__ frames,
                                               int frame_count, TileDesc* __restrict__ tiles,
                                               ShardDesc* __restrict__ shards,
-                                              int32_t* __restrict__ group_first, int tile_total) {
+                                              int32_t* __restrict__ group_first, int tile_total,
+                                              PlanInit init) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (init.kind != 0) {
+    if (t < init.queue_count) init.queues[t] = 0;
+    if (t < init.list_counts) init.list_count[t] = 0;
+    if (t < frame_count) {
+      FrameState s = {};
+      if (init.kind == 1) {
+        s.step = frames[t].step0;
+        s.use_med = frames[t].layout != 0;  // first pass: MED for colour, gradient for gray (encode.cu:812)
+      }
+      s.active = 1;
+      init.states[t] = s;
+    }
+  }
   if (t >= tile_total) return;
   int lo = 0, hi = frame_count - 1;
   while (lo < hi) {
@@ -102,12 +116,16 @@ int device_cus() {
 }
 
 void launch_plan(const FrameDesc* frames, int frame_count, TileDesc* tiles, ShardDesc* shards,
-                 int32_t* group_first, int tile_total, hipStream_t stream) {
-  if (tile_total <= 0) return;
+                 int32_t* group_first, int tile_total, hipStream_t stream, const PlanInit& init) {
+  int threads = max(tile_total, 0);
+  if (init.kind != 0) {
+    threads = max(threads, max(frame_count, max(init.queue_count, init.list_counts)));
+  }
+  if (threads <= 0) return;
   {
     FVH_TIMED("k_plan", stream);
-    hipLaunchKernelGGL(k_plan, dim3((tile_total + 255) / 256), dim3(256), 0, stream, frames,
-                       frame_count, tiles, shards, group_first, tile_total);
+    hipLaunchKernelGGL(k_plan, dim3((threads + 255) / 256), dim3(256), 0, stream, frames,
+                       frame_count, tiles, shards, group_first, tile_total, init);
   }
 }
 
