@@ -7,7 +7,7 @@ frames resident in HBM before the timed region.  Metric: luma Gpixel/s of
 encode+decode, with per-frame latency percentiles of the plain per-frame call
 beside it.
 
-Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build targets one
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: the library runs on one
 GPU (frames are independent: several GPUs would each take a share of a
 batch's frames in a process of their own).
 """

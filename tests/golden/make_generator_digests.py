@@ -1,6 +1,6 @@
 """Writes generator_digests.json: SHA-256 of the synthetic workload frames.
 
-The workloads' synthetic frames (fastvid_amd.synthetic) are this build's own
+The workloads' synthetic frames (fastvid_amd.synthetic) are this package's own
 definition; their digests freeze it, so a change to the generator (or to
 numpy's PCG64 streams) shows up as a test failure instead of silently
 changing every benchmark input.  Run from the repository root:
